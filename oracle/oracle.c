@@ -1,0 +1,433 @@
+/*
+ * CPU oracle for the finite-field NTT -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of the reference's host FFT (test/fft_host.h) and of
+ * the field arithmetic it runs on (cuda/device_field_operators.h), written
+ * to be checked against golden vectors, the DFT definition and the
+ * reference source compiled from /root/reference.  The GPU library never
+ * links or calls this file.
+ *
+ * Deliberate deviations from the reference (all are reference defects):
+ *   1. Montgomery constant: -p^-1 mod 2^32 is computed (0xE45E7FFF for
+ *      MNT4-753 Fr); the reference hard-codes 0xFFFFFFFF
+ *      (device_field_operators.h:47).
+ *   2. Final reduction: CIOS ends with a full conditional subtraction of p;
+ *      the reference's montyNormalize copies 24 bytes instead of 24 limbs
+ *      (device_field_operators.h:142,147).
+ *   3. one() is R mod p and pow() is square-and-multiply inside the
+ *      Montgomery domain; the reference returns raw 1 and leaves the domain
+ *      (device_field.h:86-91, device_field_operators.h:222-258).
+ *   4. The butterfly computes t = w * a[k+j+m]; the reference sets t = w and
+ *      multiplies w by the odd input (test/fft_host.h:44-48).  Passing
+ *      literal_butterfly = 1 restores the reference's statements exactly.
+ */
+#include "oracle.h"
+
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+static int nthreads_for(int threads) { return threads > 0 ? threads : omp_get_max_threads(); }
+#else
+static int nthreads_for(int threads) { (void)threads; return 1; }
+#endif
+
+/* MNT4-753 Fr limbs as stored in cuda/device_field.h:62-65. */
+static const uint32_t MNT4_LIMBS[24] = {
+    610172929u, 1586521054u, 752685471u, 3818738770u, 2596546032u, 1669861489u,
+    1987204260u, 1750781161u, 3411246648u, 3087994277u, 4061660573u, 2971133814u,
+    2707093405u, 2580620505u, 3902860685u, 134068517u, 1821890675u, 1589111033u,
+    1536143341u, 3086587728u, 4007841197u, 270700578u, 764593169u, 115910u};
+
+/* MNT6-753 Fr, the decimal printed at cuda/device_field.h:446. */
+static const char *MNT6_DECIMAL =
+    "41898490967918953402344214791240637128170709919953949071783502921025352812571106773"
+    "05889376379033892141807097188845847732317305749159385506969624185479639616572141632"
+    "5350064441470418137846398469611935719059908164220784476160001";
+
+/* ---------------------------------------------------------------- limbs */
+
+static int geq_n(const uint32_t *a, const uint32_t *b, int s)
+{
+    for (int i = s - 1; i >= 0; --i) {
+        if (a[i] != b[i])
+            return a[i] > b[i];
+    }
+    return 1;
+}
+
+static uint32_t add_n(uint32_t *r, const uint32_t *a, const uint32_t *b, int s)
+{
+    uint64_t c = 0;
+    for (int i = 0; i < s; ++i) {
+        c += (uint64_t)a[i] + b[i];
+        r[i] = (uint32_t)c;
+        c >>= 32;
+    }
+    return (uint32_t)c;
+}
+
+static uint32_t sub_n(uint32_t *r, const uint32_t *a, const uint32_t *b, int s)
+{
+    uint64_t borrow = 0;
+    for (int i = 0; i < s; ++i) {
+        uint64_t d = (uint64_t)a[i] - b[i] - borrow;
+        r[i] = (uint32_t)d;
+        borrow = (d >> 63) & 1;
+    }
+    return (uint32_t)borrow;
+}
+
+/* x = (2x) mod p for x < p. */
+static void dbl_mod(const or_field *f, uint32_t *x)
+{
+    uint32_t c = add_n(x, x, x, f->nlimbs);
+    if (c || geq_n(x, f->mod, f->nlimbs))
+        sub_n(x, x, f->mod, f->nlimbs);
+}
+
+static void parse_decimal(const char *s, uint32_t *out, int nlimbs)
+{
+    memset(out, 0, sizeof(uint32_t) * nlimbs);
+    for (; *s; ++s) {
+        uint64_t carry = (uint64_t)(*s - '0');
+        for (int i = 0; i < nlimbs; ++i) {
+            uint64_t v = (uint64_t)out[i] * 10u + carry;
+            out[i] = (uint32_t)v;
+            carry = v >> 32;
+        }
+    }
+}
+
+/* ---------------------------------------------------------------- fields */
+
+int or_field_init(or_field *f, int field_id)
+{
+    memset(f, 0, sizeof(*f));
+    f->id = field_id;
+    switch (field_id) {
+    case OR_FIELD_FP32:
+        f->nlimbs = 1;
+        f->mod[0] = 3u * (1u << 30) + 1u;
+        break;
+    case OR_FIELD_MNT4753_FR:
+        f->nlimbs = 24;
+        memcpy(f->mod, MNT4_LIMBS, sizeof(MNT4_LIMBS));
+        break;
+    case OR_FIELD_MNT6753_FR:
+        f->nlimbs = 24;
+        parse_decimal(MNT6_DECIMAL, f->mod, 24);
+        break;
+    default:
+        return -1;
+    }
+    /* -p^-1 mod 2^32 by Newton iteration on the low limb. */
+    uint32_t inv = 1;
+    for (int i = 0; i < 6; ++i)
+        inv *= 2u - f->mod[0] * inv;
+    f->mprime = 0u - inv;
+    /* R mod p and R^2 mod p by repeated doubling from 1. */
+    uint32_t x[OR_MAX_LIMBS] = {0};
+    x[0] = 1;
+    for (int i = 0; i < 32 * f->nlimbs; ++i)
+        dbl_mod(f, x);
+    memcpy(f->one, x, sizeof(uint32_t) * f->nlimbs);
+    for (int i = 0; i < 32 * f->nlimbs; ++i)
+        dbl_mod(f, x);
+    memcpy(f->r2, x, sizeof(uint32_t) * f->nlimbs);
+    /* 2-adicity of p - 1 */
+    uint32_t pm1[OR_MAX_LIMBS];
+    memcpy(pm1, f->mod, sizeof(pm1));
+    pm1[0] -= 1; /* p is odd, no borrow */
+    unsigned s = 0;
+    for (int i = 0; i < f->nlimbs; ++i) {
+        if (pm1[i] == 0) {
+            s += 32;
+            continue;
+        }
+        s += (unsigned)__builtin_ctz(pm1[i]);
+        break;
+    }
+    f->two_adicity = s;
+    return 0;
+}
+
+const or_field *or_field_get(int field_id)
+{
+    static or_field fields[3];
+    static int ready[3];
+    if (field_id < 0 || field_id > 2)
+        return NULL;
+    if (!ready[field_id]) {
+        or_field tmp;
+        or_field_init(&tmp, field_id);
+        fields[field_id] = tmp;
+        ready[field_id] = 1;
+    }
+    return &fields[field_id];
+}
+
+int or_is_canonical(const or_field *f, const uint32_t *a)
+{
+    return !geq_n(a, f->mod, f->nlimbs);
+}
+
+/* Scalar::add, device_field_operators.h:190-195, reducing to [0, p). */
+void or_add(const or_field *f, uint32_t *r, const uint32_t *a, const uint32_t *b)
+{
+    uint32_t c = add_n(r, a, b, f->nlimbs);
+    if (c || geq_n(r, f->mod, f->nlimbs))
+        sub_n(r, r, f->mod, f->nlimbs);
+}
+
+/* Scalar::subtract, device_field_operators.h:198-204. */
+void or_sub(const or_field *f, uint32_t *r, const uint32_t *a, const uint32_t *b)
+{
+    uint32_t borrow = sub_n(r, a, b, f->nlimbs);
+    if (borrow)
+        add_n(r, r, f->mod, f->nlimbs);
+}
+
+/* ciosMontgomeryMultiply, device_field_operators.h:151-187 (CIOS, Koc et
+ * al.), with the correct m' and a full final subtraction. */
+void or_mul(const or_field *f, uint32_t *r, const uint32_t *a, const uint32_t *b)
+{
+    const int s = f->nlimbs;
+    uint32_t t[OR_MAX_LIMBS + 2];
+    memset(t, 0, sizeof(t));
+    for (int i = 0; i < s; ++i) {
+        uint64_t C = 0, x;
+        for (int j = 0; j < s; ++j) {
+            x = (uint64_t)t[j] + (uint64_t)a[j] * b[i] + C;
+            t[j] = (uint32_t)x;
+            C = x >> 32;
+        }
+        x = (uint64_t)t[s] + C;
+        t[s] = (uint32_t)x;
+        t[s + 1] = (uint32_t)(x >> 32);
+        const uint32_t m = t[0] * f->mprime;
+        x = (uint64_t)t[0] + (uint64_t)m * f->mod[0];
+        C = x >> 32;
+        for (int j = 1; j < s; ++j) {
+            x = (uint64_t)t[j] + (uint64_t)m * f->mod[j] + C;
+            t[j - 1] = (uint32_t)x;
+            C = x >> 32;
+        }
+        x = (uint64_t)t[s] + C;
+        t[s - 1] = (uint32_t)x;
+        t[s] = t[s + 1] + (uint32_t)(x >> 32);
+    }
+    if (t[s] || geq_n(t, f->mod, s))
+        sub_n(t, t, f->mod, s);
+    memcpy(r, t, sizeof(uint32_t) * s);
+}
+
+/* Scalar::pow (device_field_operators.h:229-258) restated as
+ * square-and-multiply in the Montgomery domain. */
+void or_pow(const or_field *f, uint32_t *r, const uint32_t *a, uint64_t e)
+{
+    uint32_t acc[OR_MAX_LIMBS], base[OR_MAX_LIMBS];
+    memcpy(acc, f->one, sizeof(acc));
+    memcpy(base, a, sizeof(uint32_t) * f->nlimbs);
+    while (e) {
+        if (e & 1)
+            or_mul(f, acc, acc, base);
+        e >>= 1;
+        if (e)
+            or_mul(f, base, base, base);
+    }
+    memcpy(r, acc, sizeof(uint32_t) * f->nlimbs);
+}
+
+static void pow_big(const or_field *f, uint32_t *r, const uint32_t *a,
+                    const uint32_t *e, int elimbs)
+{
+    uint32_t acc[OR_MAX_LIMBS];
+    memcpy(acc, f->one, sizeof(acc));
+    for (int i = elimbs - 1; i >= 0; --i) {
+        for (int b = 31; b >= 0; --b) {
+            or_mul(f, acc, acc, acc);
+            if ((e[i] >> b) & 1)
+                or_mul(f, acc, acc, a);
+        }
+    }
+    memcpy(r, acc, sizeof(uint32_t) * f->nlimbs);
+}
+
+void or_inverse(const or_field *f, uint32_t *r, const uint32_t *a)
+{
+    uint32_t e[OR_MAX_LIMBS], two[OR_MAX_LIMBS] = {2};
+    sub_n(e, f->mod, two, f->nlimbs);
+    pow_big(f, r, a, e, f->nlimbs);
+}
+
+void or_to_mont(const or_field *f, uint32_t *r, const uint32_t *a)
+{
+    or_mul(f, r, a, f->r2);
+}
+
+void or_from_mont(const or_field *f, uint32_t *r, const uint32_t *a)
+{
+    uint32_t one_raw[OR_MAX_LIMBS] = {1};
+    or_mul(f, r, a, one_raw);
+}
+
+/* ------------------------------------------------------------------- FFT */
+
+/* bitreverse_host, test/fft_host.h:7-16 */
+size_t or_bitreverse(size_t n, unsigned l)
+{
+    size_t r = 0;
+    for (unsigned k = 0; k < l; ++k) {
+        r = (r << 1) | (n & 1);
+        n >>= 1;
+    }
+    return r;
+}
+
+static unsigned ilog2(size_t n)
+{
+    unsigned l = 0;
+    while (((size_t)1 << l) < n)
+        ++l;
+    return l;
+}
+
+#define EL(base, i, s) ((base) + (size_t)(i) * (size_t)(s))
+
+/* _basic_serial_radix2_FFT, test/fft_host.h:18-54 */
+void or_serial_radix2_fft(const or_field *f, uint32_t *a, size_t n,
+                          const uint32_t *omega, const uint32_t *one,
+                          int literal_butterfly)
+{
+    const int s = f->nlimbs;
+    const unsigned logn = ilog2(n);
+    uint32_t tmp[OR_MAX_LIMBS], t[OR_MAX_LIMBS], w[OR_MAX_LIMBS], w_m[OR_MAX_LIMBS];
+
+    /* in-place bit-reversal swap (fft_host.h:24-29) */
+    for (size_t k = 0; k < n; ++k) {
+        const size_t rk = or_bitreverse(k, logn);
+        if (k < rk) {
+            memcpy(tmp, EL(a, k, s), sizeof(uint32_t) * s);
+            memcpy(EL(a, k, s), EL(a, rk, s), sizeof(uint32_t) * s);
+            memcpy(EL(a, rk, s), tmp, sizeof(uint32_t) * s);
+        }
+    }
+
+    size_t m = 1; /* invariant: m = 2^{stage-1} (fft_host.h:31) */
+    for (unsigned st = 1; st <= logn; ++st) {
+        or_pow(f, w_m, omega, n / (2 * m)); /* fft_host.h:35-36 */
+        for (size_t k = 0; k < n; k += 2 * m) {
+            memcpy(w, one, sizeof(uint32_t) * s);
+            for (size_t j = 0; j < m; ++j) {
+                uint32_t *lo = EL(a, k + j, s);
+                uint32_t *hi = EL(a, k + j + m, s);
+                if (literal_butterfly) {
+                    /* fft_host.h:44-48 as written */
+                    memcpy(t, w, sizeof(uint32_t) * s);
+                    or_mul(f, w, w, hi);
+                } else {
+                    or_mul(f, t, w, hi);
+                }
+                or_sub(f, hi, lo, t);
+                or_add(f, lo, lo, t);
+                or_mul(f, w, w, w_m);
+            }
+        }
+        m *= 2;
+    }
+}
+
+/* Phase 1 of _basic_parallel_radix2_FFT_inner, test/fft_host.h:76-102, for
+ * the partial sums j in [j_begin, j_end).  tmp rows are (n >> log_cpus)
+ * elements long and are written (not accumulated) here. */
+static void phase1_rows(const or_field *f, const uint32_t *a, size_t n,
+                        const uint32_t *omega, unsigned log_cpus,
+                        size_t j_begin, size_t j_end, uint32_t *tmp, int threads)
+{
+    const int s = f->nlimbs;
+    const size_t num_cpus = (size_t)1 << log_cpus;
+    const unsigned log_m = ilog2(n);
+    const size_t rows = (size_t)1 << (log_m - log_cpus);
+    long jb = (long)j_begin, je = (long)j_end;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads_for(threads))
+    for (long jj = jb; jj < je; ++jj) {
+        const size_t j = (size_t)jj;
+        uint32_t omega_j[OR_MAX_LIMBS], omega_step[OR_MAX_LIMBS], elt[OR_MAX_LIMBS];
+        uint32_t temp[OR_MAX_LIMBS];
+        uint32_t *row = tmp + (j - j_begin) * rows * s;
+        memset(row, 0, sizeof(uint32_t) * rows * s); /* FieldT::zero(), fft_host.h:73 */
+        or_pow(f, omega_j, omega, j);
+        or_pow(f, omega_step, omega, j << (log_m - log_cpus));
+        memcpy(elt, f->one, sizeof(uint32_t) * s); /* FieldT::one(), fft_host.h:85 */
+        for (size_t i = 0; i < rows; ++i) {
+            for (size_t q = 0; q < num_cpus; ++q) {
+                /* invariant: elt = omega^(j * idx) */
+                const size_t idx = (i + (q << (log_m - log_cpus))) % ((size_t)1 << log_m);
+                or_mul(f, temp, EL(a, idx, s), elt);
+                or_add(f, EL(row, i, s), EL(row, i, s), temp);
+                or_mul(f, elt, elt, omega_step);
+            }
+            or_mul(f, elt, elt, omega_j);
+        }
+    }
+}
+
+void or_parallel_phase1_sample(const or_field *f, const uint32_t *a, size_t n,
+                               const uint32_t *omega, unsigned log_cpus,
+                               size_t j_begin, size_t j_end, uint32_t *tmp_out,
+                               int threads)
+{
+    phase1_rows(f, a, n, omega, log_cpus, j_begin, j_end, tmp_out, threads);
+}
+
+/* _basic_parallel_radix2_FFT_inner, test/fft_host.h:56-118 */
+void or_parallel_radix2_fft_inner(const or_field *f, uint32_t *a, size_t n,
+                                  const uint32_t *omega, unsigned log_cpus,
+                                  const uint32_t *one, int literal_butterfly,
+                                  int threads)
+{
+    const int s = f->nlimbs;
+    const size_t num_cpus = (size_t)1 << log_cpus;
+    const unsigned log_m = ilog2(n);
+
+    if (log_m < log_cpus) { /* fft_host.h:64-68 */
+        or_serial_radix2_fft(f, a, n, omega, one, literal_butterfly);
+        return;
+    }
+    const size_t rows = (size_t)1 << (log_m - log_cpus);
+    uint32_t *tmp = (uint32_t *)malloc(sizeof(uint32_t) * s * rows * num_cpus);
+    if (!tmp)
+        abort();
+
+    phase1_rows(f, a, n, omega, log_cpus, 0, num_cpus, tmp, threads);
+
+    uint32_t omega_num_cpus[OR_MAX_LIMBS];
+    or_pow(f, omega_num_cpus, omega, num_cpus); /* fft_host.h:103 */
+    const long nc = (long)num_cpus;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads_for(threads))
+    for (long j = 0; j < nc; ++j) /* fft_host.h:104-108 */
+        or_serial_radix2_fft(f, tmp + (size_t)j * rows * s, rows, omega_num_cpus, one,
+                             literal_butterfly);
+
+#pragma omp parallel for num_threads(nthreads_for(threads))
+    for (long i = 0; i < nc; ++i) /* fft_host.h:110-117 */
+        for (size_t j = 0; j < rows; ++j)
+            memcpy(EL(a, (j << log_cpus) + (size_t)i, s), tmp + ((size_t)i * rows + j) * s,
+                   sizeof(uint32_t) * s);
+    free(tmp);
+}
+
+void or_inverse_fft(const or_field *f, uint32_t *a, size_t n, const uint32_t *omega)
+{
+    const int s = f->nlimbs;
+    uint32_t winv[OR_MAX_LIMBS], nm[OR_MAX_LIMBS] = {0}, nraw[OR_MAX_LIMBS] = {0}, ninv[OR_MAX_LIMBS];
+    or_inverse(f, winv, omega);
+    or_serial_radix2_fft(f, a, n, winv, f->one, 0);
+    nraw[0] = (uint32_t)n; /* n <= 2^30 */
+    or_to_mont(f, nm, nraw);
+    or_inverse(f, ninv, nm);
+    for (size_t i = 0; i < n; ++i)
+        or_mul(f, EL(a, i, s), EL(a, i, s), ninv);
+}

@@ -1,0 +1,151 @@
+"""ctypes access to the CPU oracle (oracle/liboracle.so) and to the
+reference-source harness (oracle/_ref/libref_fft_host.so).  Test-side only."""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libref_fft_host.so")
+
+FIELD_IDS = {"fp32": 0, "mnt4753_fr": 1, "mnt6753_fr": 2}
+NLIMBS = {0: 1, 1: 24, 2: 24}
+
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+
+
+class OrField(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_int), ("nlimbs", ctypes.c_int),
+                ("mod", ctypes.c_uint32 * 24), ("mprime", ctypes.c_uint32),
+                ("one", ctypes.c_uint32 * 24), ("r2", ctypes.c_uint32 * 24),
+                ("two_adicity", ctypes.c_uint)]
+
+
+def _p(a):
+    assert a.dtype == np.uint32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_u32p)
+
+
+class Oracle:
+    def __init__(self, path=ORACLE_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError("%s is not built (run: make -C oracle)" % path)
+        L = ctypes.CDLL(path)
+        L.or_field_get.restype = ctypes.POINTER(OrField)
+        L.or_field_get.argtypes = [ctypes.c_int]
+        for nm in ("or_add", "or_sub", "or_mul"):
+            getattr(L, nm).argtypes = [ctypes.POINTER(OrField), _u32p, _u32p, _u32p]
+        L.or_pow.argtypes = [ctypes.POINTER(OrField), _u32p, _u32p, ctypes.c_uint64]
+        L.or_inverse.argtypes = [ctypes.POINTER(OrField), _u32p, _u32p]
+        L.or_to_mont.argtypes = [ctypes.POINTER(OrField), _u32p, _u32p]
+        L.or_from_mont.argtypes = [ctypes.POINTER(OrField), _u32p, _u32p]
+        L.or_bitreverse.restype = ctypes.c_size_t
+        L.or_bitreverse.argtypes = [ctypes.c_size_t, ctypes.c_uint]
+        L.or_serial_radix2_fft.argtypes = [ctypes.POINTER(OrField), _u32p, ctypes.c_size_t, _u32p, _u32p, ctypes.c_int]
+        L.or_parallel_radix2_fft_inner.argtypes = [ctypes.POINTER(OrField), _u32p, ctypes.c_size_t, _u32p,
+                                                   ctypes.c_uint, _u32p, ctypes.c_int, ctypes.c_int]
+        L.or_parallel_phase1_sample.argtypes = [ctypes.POINTER(OrField), _u32p, ctypes.c_size_t, _u32p,
+                                                ctypes.c_uint, ctypes.c_size_t, ctypes.c_size_t, _u32p, ctypes.c_int]
+        L.or_inverse_fft.argtypes = [ctypes.POINTER(OrField), _u32p, ctypes.c_size_t, _u32p]
+        self.lib = L
+
+    def field(self, fid):
+        return self.lib.or_field_get(fid)
+
+    def params(self, fid):
+        f = self.field(fid).contents
+        L = f.nlimbs
+        return dict(nlimbs=L, mod=np.array(f.mod[:L], np.uint32), mprime=f.mprime,
+                    one=np.array(f.one[:L], np.uint32), r2=np.array(f.r2[:L], np.uint32),
+                    two_adicity=f.two_adicity)
+
+    def _binop(self, name, fid, a, b):
+        a = np.ascontiguousarray(a, np.uint32)
+        b = np.ascontiguousarray(b, np.uint32)
+        out = np.zeros_like(a)
+        f = self.field(fid)
+        fn = getattr(self.lib, name)
+        for i in range(a.shape[0]):
+            fn(f, _p(out[i]), _p(a[i]), _p(b[i]))
+        return out
+
+    def add(self, fid, a, b): return self._binop("or_add", fid, a, b)
+    def sub(self, fid, a, b): return self._binop("or_sub", fid, a, b)
+    def mul(self, fid, a, b): return self._binop("or_mul", fid, a, b)
+
+    def pow(self, fid, a, exps):
+        a = np.ascontiguousarray(a, np.uint32)
+        out = np.zeros_like(a)
+        f = self.field(fid)
+        for i in range(a.shape[0]):
+            self.lib.or_pow(f, _p(out[i]), _p(a[i]), int(exps[i]))
+        return out
+
+    def unary(self, name, fid, a):
+        a = np.ascontiguousarray(a, np.uint32).reshape(-1, NLIMBS[fid])
+        out = np.zeros_like(a)
+        f = self.field(fid)
+        for i in range(a.shape[0]):
+            getattr(self.lib, name)(f, _p(out[i]), _p(a[i]))
+        return out
+
+    def serial_fft(self, fid, x, omega, literal=False):
+        a = np.ascontiguousarray(x, np.uint32).copy()
+        one = self.params(fid)["one"]
+        self.lib.or_serial_radix2_fft(self.field(fid), _p(a), a.shape[0],
+                                      _p(np.ascontiguousarray(omega, np.uint32)), _p(one), int(literal))
+        return a
+
+    def parallel_fft(self, fid, x, omega, log_cpus, literal=False, threads=0):
+        a = np.ascontiguousarray(x, np.uint32).copy()
+        one = self.params(fid)["one"]
+        self.lib.or_parallel_radix2_fft_inner(self.field(fid), _p(a), a.shape[0],
+                                              _p(np.ascontiguousarray(omega, np.uint32)), log_cpus,
+                                              _p(one), int(literal), threads)
+        return a
+
+    def inverse_fft(self, fid, x, omega):
+        a = np.ascontiguousarray(x, np.uint32).copy()
+        self.lib.or_inverse_fft(self.field(fid), _p(a), a.shape[0], _p(np.ascontiguousarray(omega, np.uint32)))
+        return a
+
+
+class RefHarness:
+    """The reference's test/fft_host.h compiled over the oracle's arithmetic."""
+
+    def __init__(self, path=REF_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        L = ctypes.CDLL(path)
+        L.ref_serial_radix2_fft.argtypes = [ctypes.c_int, _u32p, ctypes.c_size_t, _u32p]
+        L.ref_parallel_radix2_fft_inner.argtypes = [ctypes.c_int, _u32p, ctypes.c_size_t, _u32p, ctypes.c_uint]
+        L.ref_scalar_add.argtypes = [_u32p, _u32p, _u32p]
+        L.ref_scalar_sub.argtypes = [_u32p, _u32p, _u32p]
+        L.ref_scalar_modulus.argtypes = [_u32p]
+        self.lib = L
+
+    def serial_fft(self, fid, x, omega):
+        a = np.ascontiguousarray(x, np.uint32).copy()
+        assert self.lib.ref_serial_radix2_fft(fid, _p(a), a.shape[0], _p(np.ascontiguousarray(omega, np.uint32))) == 0
+        return a
+
+    def parallel_fft(self, fid, x, omega, log_cpus):
+        a = np.ascontiguousarray(x, np.uint32).copy()
+        assert self.lib.ref_parallel_radix2_fft_inner(fid, _p(a), a.shape[0],
+                                                      _p(np.ascontiguousarray(omega, np.uint32)), log_cpus) == 0
+        return a
+
+    def scalar_op(self, name, a, b):
+        a = np.ascontiguousarray(a, np.uint32)
+        b = np.ascontiguousarray(b, np.uint32)
+        out = np.zeros_like(a)
+        fn = getattr(self.lib, "ref_scalar_" + name)
+        for i in range(a.shape[0]):
+            fn(_p(out[i]), _p(a[i]), _p(b[i]))
+        return out
+
+    def modulus(self):
+        out = np.zeros(24, np.uint32)
+        self.lib.ref_scalar_modulus(_p(out))
+        return out

@@ -1,0 +1,223 @@
+"""Pins the CPU oracle (oracle/oracle.c) before anything is checked against it.
+
+Each check is chosen so that a plausible slip fails it:
+  * golden vectors from tests/golden/gen_golden.py (Python big ints, the DFT
+    definition evaluated in O(n^2)) -- pins the field ops and the transform;
+  * the known-answer values of the reference's own operator test
+    (cuda/device_field_operator_test.cpp:88-100,222-320);
+  * the reference's test/fft_host.h, compiled unmodified over the oracle's
+    arithmetic (oracle/_ref), against the oracle run with the reference's
+    literal butterfly -- pins the loop structure (bit reversal, stage order,
+    running twiddle, C-way split, scatter) of the restatement;
+  * the reference's Scalar add/subtract (cuda/device_field_operators.h:190-204);
+  * closed forms: delta, constants, linearity, convolution theorem, inverse.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from gpusnarks_amd import synth
+from tests.oracle_lib import FIELD_IDS, NLIMBS, Oracle, RefHarness, REF_SO
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return Oracle()
+
+
+def golden(kind, name):
+    return np.load(os.path.join(GOLDEN, "%s_%s.npz" % (kind, name)))
+
+
+def ints(arr):
+    return [synth.to_int(r) for r in np.asarray(arr).reshape(arr.shape[0], -1)]
+
+
+def limbs(vals, L):
+    return np.stack([synth.from_int(v, L) for v in vals])
+
+
+FIELDS = list(FIELD_IDS.items())
+
+
+@pytest.mark.parametrize("name,fid", FIELDS)
+def test_constants_match_golden(orc, name, fid):
+    g = golden("field", name)
+    p = orc.params(fid)
+    assert np.array_equal(p["mod"], g["modulus"])
+    assert np.array_equal(p["one"], g["one"])
+    assert np.array_equal(p["r2"], g["r2"])
+    assert p["mprime"] == int(g["mprime"][0])
+    assert p["two_adicity"] == {0: 30, 1: 15, 2: 30}[fid]
+
+
+def test_mprime_closed_form(orc):
+    # m' * p == -1 mod 2^32; the reference's 0xFFFFFFFF (device_field_operators.h:47) is not it
+    for fid in (0, 1, 2):
+        p = orc.params(fid)
+        assert (int(p["mprime"]) * int(p["mod"][0])) % 2**32 == 2**32 - 1
+    assert orc.params(1)["mprime"] == 0xE45E7FFF
+
+
+@pytest.mark.parametrize("name,fid", FIELDS)
+def test_field_ops_match_golden(orc, name, fid):
+    g = golden("field", name)
+    a, b = g["a"], g["b"]
+    assert np.array_equal(orc.add(fid, a, b), g["add"])
+    assert np.array_equal(orc.sub(fid, a, b), g["sub"])
+    assert np.array_equal(orc.mul(fid, a, b), g["mont_mul"])
+    assert np.array_equal(orc.pow(fid, a, g["exp"]), g["mont_pow"])
+
+
+@pytest.mark.parametrize("fid", [1, 2])
+def test_reference_known_answers(orc, fid):
+    """KATs of cuda/device_field_operator_test.cpp, in the Montgomery domain."""
+    L = NLIMBS[fid]
+
+    def m(v):  # to Montgomery form
+        return orc.unary("or_to_mont", fid, synth.from_int(v, L))
+
+    def val(x):  # out of Montgomery form, as an int
+        return synth.to_int(orc.unary("or_from_mont", fid, x)[0])
+
+    one = orc.params(fid)["one"][None, :]
+    # testAdd :222-231
+    assert val(orc.add(fid, m(1234), m(1234))) == 2468
+    # test_subtract :233-244
+    assert val(orc.sub(fid, m(1234), m(1234))) == 0
+    assert np.array_equal(orc.sub(fid, m(1235), m(1234)), one)
+    # testMultiply :246-257
+    f1 = orc.mul(fid, m(1234), m(1234))
+    assert val(f1) == 1522756
+    assert val(orc.mul(fid, f1, m(1234))) == 1879080904
+    # testPow :278-297 (2^35 lands in limb 1, not limb 22: limbs are little-endian)
+    assert np.array_equal(orc.pow(fid, m(2), [0]), one)
+    assert val(orc.pow(fid, m(2), [2])) == 4
+    assert val(orc.pow(fid, m(4), [10])) == 1048576
+    assert val(orc.pow(fid, m(2), [20])) == 1048576
+    r35 = orc.unary("or_from_mont", fid, orc.pow(fid, m(2), [35]))[0]
+    assert r35[1] == 8 and r35[0] == 0 and not r35[2:].any()
+    # testConstructor :301-320: Scalar(1) == one()
+    assert np.array_equal(m(1), one)
+    # calculateModPrime :88-100: R * R^-1 == 1
+    assert val(one) == 1
+
+
+def test_bitreverse(orc):
+    for l in range(0, 13):
+        seen = set()
+        for k in range(1 << l):
+            r = orc.lib.or_bitreverse(k, l)
+            assert r == int(format(k, "0%db" % l)[::-1] or "0", 2) if l else r == 0
+            assert orc.lib.or_bitreverse(r, l) == k
+            seen.add(r)
+        assert len(seen) == 1 << l
+
+
+@pytest.mark.parametrize("name,fid", FIELDS)
+def test_serial_fft_matches_dft_golden(orc, name, fid):
+    g = golden("ntt", name)
+    sizes = sorted(int(k[1:].split("_")[0]) for k in g.files if k.endswith("_input"))
+    assert sizes
+    for n in sizes:
+        x, om = g["n%d_input" % n], g["n%d_omega" % n]
+        assert np.array_equal(orc.serial_fft(fid, x, om), g["n%d_forward" % n]), n
+        assert np.array_equal(orc.inverse_fft(fid, x, om), g["n%d_inverse" % n]), n
+
+
+@pytest.mark.parametrize("name,fid", FIELDS)
+def test_parallel_equals_serial(orc, name, fid):
+    g = golden("ntt", name)
+    for n in (8, 64, 256):
+        if "n%d_input" % n not in g.files:
+            continue
+        x, om = g["n%d_input" % n], g["n%d_omega" % n]
+        ref = g["n%d_forward" % n]
+        for log_cpus in (0, 1, 2, 3, 10):
+            out = orc.parallel_fft(fid, x, om, log_cpus)
+            assert np.array_equal(out, ref), (n, log_cpus)
+
+
+def test_parallel_log_cpus_10_full_split(orc):
+    # n >= 2^10 so the harness setting log_cpus = 10 (test/main.cpp:71) takes the split path
+    fid = 0
+    g = golden("ntt", "fp32")
+    x, om = g["n1024_input"], g["n1024_omega"]
+    assert np.array_equal(orc.parallel_fft(fid, x, om, 10), g["n1024_forward"])
+    assert np.array_equal(orc.parallel_fft(fid, x, om, 10, threads=1), g["n1024_forward"])
+
+
+ref_missing = not os.path.exists(REF_SO)
+
+
+@pytest.mark.skipif(ref_missing, reason="oracle/_ref not built (reference tree absent)")
+@pytest.mark.parametrize("name,fid", FIELDS)
+def test_literal_butterfly_matches_reference_source(orc, name, fid):
+    """The oracle with the reference's butterfly equals test/fft_host.h itself."""
+    ref = RefHarness()
+    g = golden("ntt", name)
+    for n in (2, 8, 64, 256):
+        if "n%d_input" % n not in g.files:
+            continue
+        x, om = g["n%d_input" % n], g["n%d_omega" % n]
+        assert np.array_equal(orc.serial_fft(fid, x, om, literal=True), ref.serial_fft(fid, x, om)), n
+        for log_cpus in (1, 3):
+            assert np.array_equal(orc.parallel_fft(fid, x, om, log_cpus, literal=True),
+                                  ref.parallel_fft(fid, x, om, log_cpus)), (n, log_cpus)
+    # and the corrected butterfly is a different (and the only DFT-correct) transform
+    x, om = g["n8_input"], g["n8_omega"]
+    assert not np.array_equal(orc.serial_fft(fid, x, om, literal=True), g["n8_forward"])
+
+
+@pytest.mark.skipif(ref_missing, reason="oracle/_ref not built (reference tree absent)")
+def test_reference_scalar_add_sub(orc):
+    """Reference Scalar add/sub (device_field_operators.h:190-204) == oracle, MNT4-753 Fr."""
+    ref = RefHarness()
+    fid = 1
+    assert np.array_equal(ref.modulus(), orc.params(fid)["mod"])
+    p = synth.to_int(orc.params(fid)["mod"])
+    a = synth.uniform_residues(11, 300, p, 24)
+    b = synth.uniform_residues(12, 300, p, 24)
+    assert np.array_equal(ref.scalar_op("add", a, b), orc.add(fid, a, b))
+    assert np.array_equal(ref.scalar_op("sub", a, b), orc.sub(fid, a, b))
+
+
+@pytest.mark.parametrize("fid,n", [(0, 64), (0, 256), (2, 16), (1, 16)])
+def test_closed_forms(orc, fid, n):
+    L = NLIMBS[fid]
+    p = synth.to_int(orc.params(fid)["mod"])
+    R = 1 << (32 * L)
+    qnr = {0: 5, 1: 13, 2: 11}[fid]
+    w = pow(qnr, (p - 1) // n, p)
+    om = synth.from_int(w * R % p, L)
+    one = orc.params(fid)["one"]
+    # omega^n == 1 and omega^(n/2) == -1
+    assert np.array_equal(orc.pow(fid, om[None], [n])[0], one)
+    assert synth.to_int(orc.pow(fid, om[None], [n // 2])[0]) == p - R % p
+    # delta -> all equal (the representation of 1 maps to itself everywhere)
+    delta = np.zeros((n, L), np.uint32)
+    delta[0] = one
+    assert (orc.serial_fft(fid, delta, om) == one).all()
+    # constant c -> (n*c, 0, ..., 0)
+    c = synth.uniform_residues(5, 1, p, L)[0]
+    const = np.tile(c, (n, 1))
+    out = ints(orc.serial_fft(fid, const, om))
+    assert out[0] == n * synth.to_int(c) % p and all(v == 0 for v in out[1:])
+    # linearity
+    x = synth.uniform_residues(6, n, p, L)
+    y = synth.uniform_residues(7, n, p, L)
+    lhs = orc.serial_fft(fid, orc.add(fid, x, y), om)
+    rhs = orc.add(fid, orc.serial_fft(fid, x, om), orc.serial_fft(fid, y, om))
+    assert np.array_equal(lhs, rhs)
+    # convolution theorem: NTT(x (*) y) == NTT(x) . NTT(y) (Montgomery product)
+    xi, yi = ints(x), ints(y)
+    rinv = pow(R, -1, p)
+    conv = [sum(xi[j] * yi[(k - j) % n] for j in range(n)) * rinv % p for k in range(n)]
+    lhs = orc.serial_fft(fid, limbs(conv, L), om)
+    rhs = orc.mul(fid, orc.serial_fft(fid, x, om), orc.serial_fft(fid, y, om))
+    assert np.array_equal(lhs, rhs)
+    # inverse(forward(x)) == x
+    assert np.array_equal(orc.inverse_fft(fid, orc.serial_fft(fid, x, om), om), x)
