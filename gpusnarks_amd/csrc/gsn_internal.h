@@ -1,0 +1,63 @@
+// Internal declarations shared by the runtime's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <mutex>
+#include <string>
+
+#define GSN_TRY(expr)                  \
+    do {                               \
+        const int gsn_st_ = (expr);    \
+        if (gsn_st_ != 0)              \
+            return gsn_st_;            \
+    } while (0)
+
+namespace gsn {
+
+int set_error(int status, const std::string &msg);
+int hip_status(hipError_t e, const char *what);
+int take_injected_fault();
+int ensure_device();
+size_t element_bytes(int field);
+int two_adicity(int field);
+int log2_exact(size_t n, unsigned *log_n);
+
+// Owning device allocation (hipMalloc / hipFree).
+struct DeviceBuffer {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer &) = delete;
+    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+    DeviceBuffer(DeviceBuffer &&o) noexcept : ptr(o.ptr), bytes(o.bytes)
+    {
+        o.ptr = nullptr;
+        o.bytes = 0;
+    }
+    DeviceBuffer &operator=(DeviceBuffer &&o) noexcept;
+    ~DeviceBuffer();
+    int alloc(size_t n);  // grows; contents are not preserved
+};
+
+// A transform plan: field, size, direction, twiddles and workspace.
+struct PlanBase {
+    int field = 0;
+    unsigned log_n = 0;
+    int inverse = 0;
+    int num_passes = 0;
+    size_t device_bytes = 0;
+    std::mutex host_mutex;  // serialises the host entry point's buffer
+    DeviceBuffer host_buf;
+    virtual ~PlanBase() = default;
+    virtual int execute(const void *in, void *out, hipStream_t s) = 0;
+};
+
+int create_plan(PlanBase **out, int field, unsigned log_n, const void *omega, int inverse);
+void clear_schedules();
+void clear_plan_cache();
+int cached_plan(PlanBase **out, int field, unsigned log_n, const void *omega, int inverse);
+
+}  // namespace gsn

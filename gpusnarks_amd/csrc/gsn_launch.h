@@ -1,0 +1,72 @@
+// Kernel argument types and launch entry points.  The kernels live in one
+// translation unit per field (kernels_*.hip) so the large 753-bit kernels
+// compile in parallel with the rest.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "field_core.h"
+
+namespace gsn {
+
+enum : uint32_t {
+    PASS_SCALE = 1u,  // multiply outputs by `scale` (n^-1 for the inverse)
+    PASS_CANON = 2u,  // reduce outputs to [0, p)
+};
+
+// Everything one Stockham pass of one transform needs.
+template <class F>
+struct PassArgs {
+    const typename F::elem_t *in;
+    typename F::elem_t *out;
+    const typename F::tw_t *tw_col;
+    const typename F::tw_t *tw_pass;
+    const typename F::tw_t *tw_pass_lo;
+    uint32_t log_n;
+    uint32_t log_ns;
+    uint32_t col_shift;
+    uint32_t lo_bits;
+    uint32_t flags;
+    typename F::tw_t scale;
+};
+
+// A pass plus its tile shape and its first tile within a multi-job launch.
+template <class F>
+struct PassJob {
+    PassArgs<F> a;
+    uint32_t log_r;
+    uint32_t log_w;
+    uint32_t first_tile;
+    uint32_t num_tiles;
+};
+
+// One Stockham pass: radix 2^log_r over 2^log_w columns per workgroup with
+// radix-2^log_e register steps.
+struct PassCfg {
+    int log_r, log_w, log_e;
+};
+
+// 753-bit tiles: at most 2^9 elements (48 KiB of LDS), 256 threads.
+constexpr int T753_LOG_TILE = 9;
+constexpr int T753_THREADS = 256;
+
+// Launches one pass.  For the 32-bit field the tile shape must be one of
+// the compiled instantiations (pass_supported); 753-bit shapes are runtime.
+template <class F>
+bool pass_supported(const PassCfg &cfg);
+template <class F>
+void launch_pass(const PassJob<F> &job, const PassCfg &cfg, hipStream_t stream);
+// Launches one pass of many transforms sharing cfg (32-bit) or any shapes
+// (753-bit): jobs and tile_job are device arrays; total_tiles blocks.
+template <class F>
+void launch_pass_multi(const PassJob<F> *d_jobs, const uint32_t *d_tile_job, uint32_t total_tiles, const PassCfg &cfg,
+                       hipStream_t stream);
+
+template <class F>
+void launch_scale(const typename F::elem_t *in, typename F::elem_t *out, typename F::tw_t s, size_t n,
+                  int do_mul, hipStream_t stream);
+
+template <class F>
+void launch_build_tw753(TW753 *out, const E753 *hi, const TW753 *lo, uint32_t h, size_t n, hipStream_t stream);
+
+}  // namespace gsn

@@ -1,0 +1,43 @@
+// Pass and table kernels of the 753-bit field MNT6-753 Fr.
+#include "ntt_kernels.h"
+
+namespace gsn {
+
+template <>
+bool pass_supported<FMnt6>(const PassCfg &c)
+{
+    return c.log_e == 1 && c.log_r >= 1 && c.log_w >= 0 && c.log_r + c.log_w <= T753_LOG_TILE;
+}
+
+template <>
+void launch_pass<FMnt6>(const PassJob<FMnt6> &job, const PassCfg &c, hipStream_t st)
+{
+    const size_t grid = ((size_t)1 << (job.a.log_n - c.log_r)) >> c.log_w;
+    hipLaunchKernelGGL(ntt753_kernel<FMnt6>, dim3((unsigned)grid), dim3(T753_THREADS), 0, st, job,
+                       (const PassJob<FMnt6> *)nullptr, (const uint32_t *)nullptr);
+}
+
+template <>
+void launch_pass_multi<FMnt6>(const PassJob<FMnt6> *jobs, const uint32_t *tile_job, uint32_t total_tiles,
+                             const PassCfg &c, hipStream_t st)
+{
+    (void)c;
+    PassJob<FMnt6> unused{};
+    hipLaunchKernelGGL(ntt753_kernel<FMnt6>, dim3(total_tiles), dim3(T753_THREADS), 0, st, unused, jobs, tile_job);
+}
+
+template <>
+void launch_scale<FMnt6>(const E753 *in, E753 *out, TW753 s, size_t n, int do_mul, hipStream_t st)
+{
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(scale_kernel<FMnt6>, dim3(blocks), dim3(256), 0, st, in, out, s, n, do_mul);
+}
+
+template <>
+void launch_build_tw753<FMnt6>(TW753 *out, const E753 *hi, const TW753 *lo, uint32_t h, size_t n, hipStream_t st)
+{
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(build_tw753_kernel<FMnt6>, dim3(blocks), dim3(256), 0, st, out, hi, lo, h, n);
+}
+
+}  // namespace gsn

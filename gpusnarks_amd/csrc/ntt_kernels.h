@@ -1,0 +1,383 @@
+// NTT kernels for gfx950: Stockham passes.
+//
+// A transform of size n = 2^L runs as a sequence of passes.  Pass p has
+// radix R = 2^LOG_R and stride Ns = 2^log_ns (the product of the radices of
+// the passes before it).  With nR = n / R, for every j in [0, nR):
+//
+//   x[r]  = in[j + r*nR] * w_n^(r*k*n/(Ns*R)),  k = j mod Ns   (pass twiddle)
+//   X     = DFT_R(x)                                         (size-R DFT)
+//   out[(j/Ns)*Ns*R + k + r*Ns] = X[r]
+//
+// which sorts itself: natural-order input gives natural-order output, so the
+// bit-reversal permutation of the reference (cuda/fft_kernel.cu:41-45,
+// test/fft_host.h:24-29) disappears into the passes' index maps.
+//
+// A workgroup takes W consecutive j (a tile of R rows by W columns).  The
+// size-R DFT of each column is itself a Stockham sequence of radix-E steps:
+// each thread holds E elements in registers, does a radix-2 DFT_E on them,
+// and exchanges through LDS between steps.  The first step reads global
+// memory and the last writes it directly when the output rows are
+// W-contiguous (Ns >= W); otherwise (the first pass) the tile is transposed
+// through LDS so that the global store is linear.
+//
+// Two kernel families share that structure:
+//   * ntt_pass_kernel<F32, LOG_R, LOG_W, LOG_E>: compile-time tile shape,
+//     radix-16 register steps (32-bit field);
+//   * ntt753_kernel<F>: runtime tile shape (R*W <= 512, 256 threads, radix-2
+//     steps) for the 753-bit fields, whose cost is the multiplier, not the
+//     indexing.
+// Each has a multi-job form: blockIdx -> (job, tile) through a tile map, so
+// one launch can run a pass of many transforms of different sizes.
+//
+// Twiddles are device-resident tables built once per plan:
+//   column table: entry (x << col_shift) holds w_R^x;
+//   pass table:   either a full table of w_n^e (e < n), or, when
+//                 tw_pass_lo is set, a two-level split
+//                 w_n^e = hi[e >> lo_bits] * lo[e & (2^lo_bits - 1)].
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "gsn_launch.h"
+
+namespace gsn {
+
+__host__ __device__ constexpr int brev_bits(int v, int bits)
+{
+    int r = 0;
+    for (int i = 0; i < bits; ++i)
+        r |= ((v >> i) & 1) << (bits - 1 - i);
+    return r;
+}
+
+template <class F>
+__device__ __forceinline__ typename F::elem_t pass_twiddle(const typename F::elem_t &x, uint32_t e,
+                                                          const PassArgs<F> &a)
+{
+    if (a.tw_pass_lo == nullptr)
+        return F::mul(x, F::load_tw(a.tw_pass, e));
+    const uint32_t hi = e >> a.lo_bits, lo = e & ((1u << a.lo_bits) - 1u);
+    typename F::elem_t y = x;
+    if (hi)
+        y = F::mul(y, F::load_tw(a.tw_pass, hi));
+    if (lo)
+        y = F::mul(y, F::load_tw(a.tw_pass_lo, lo));
+    return y;
+}
+
+// Natural-order radix-2 DFT of ES = 2^LOG_ES register elements x[OFF..OFF+ES),
+// root w_ES = w_R^(R/ES); w_{2m}^jx = w_R^(jx * R / 2m) from the column table.
+template <class F, int LOG_R, int LOG_ES, int OFF, int N>
+__device__ __forceinline__ void dft_regs(typename F::elem_t (&x)[N], const PassArgs<F> &a)
+{
+    constexpr int ES = 1 << LOG_ES;
+    typename F::elem_t y[ES];
+#pragma unroll
+    for (int i = 0; i < ES; ++i)
+        y[brev_bits(i, LOG_ES)] = x[OFF + i];
+#pragma unroll
+    for (int s = 1; s <= LOG_ES; ++s) {
+        const int m = 1 << (s - 1);
+#pragma unroll
+        for (int kb = 0; kb < ES; kb += 2 * m) {
+#pragma unroll
+            for (int jx = 0; jx < m; ++jx) {
+                typename F::elem_t t = y[kb + jx + m];
+                if (jx)
+                    t = F::mul(t, F::load_tw(a.tw_col, (size_t)(jx << (LOG_R - s)) << a.col_shift));
+                y[kb + jx + m] = F::sub(y[kb + jx], t);
+                y[kb + jx] = F::add(y[kb + jx], t);
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < ES; ++i)
+        x[OFF + i] = y[i];
+}
+
+template <class F, int LOG_R, int LOG_ES, int N, int G>
+__device__ __forceinline__ void dft_groups(typename F::elem_t (&x)[N], const PassArgs<F> &a)
+{
+    if constexpr (G > 0) {
+        dft_groups<F, LOG_R, LOG_ES, N, G - 1>(x, a);
+        dft_regs<F, LOG_R, LOG_ES, (G - 1) << LOG_ES, N>(x, a);
+    }
+}
+
+// ------------------------------------------------ compile-time tile shape
+
+// Column steps: step 0 has radix E0 = 2^LOG_E0 (LOG_R mod LOG_E, or LOG_E
+// when that is 0) and every later step radix E.  In a step of radix ES each
+// thread owns G = E/ES groups; group g is column index jj' = jj + g*(R/E)
+// of that step and holds rows jj' + e*(R/ES).
+template <class F, int LOG_R, int LOG_W, int LOG_E>
+__device__ __forceinline__ void pass_body(const PassArgs<F> &a, size_t blk)
+{
+    using elem = typename F::elem_t;
+    static_assert(LOG_E <= LOG_R, "register radix larger than the column");
+    constexpr int R = 1 << LOG_R, W = 1 << LOG_W, E = 1 << LOG_E;
+    constexpr int LOG_E0 = (LOG_R % LOG_E) ? (LOG_R % LOG_E) : LOG_E;
+    constexpr int E0 = 1 << LOG_E0, G0 = E / E0;
+    constexpr int TILE = R * W, NT = TILE / E;
+    constexpr int STEPS = 1 + (LOG_R - LOG_E0) / LOG_E;
+    constexpr int STRIDE = TILE;  // LDS words between limbs (multi-limb fields)
+    __shared__ uint32_t lds[TILE * F::LDS_WORDS_PER_ELEM];
+
+    const int t = threadIdx.x;
+    const uint32_t L = a.log_n;
+    const size_t nR = (size_t)1 << (L - LOG_R);
+    const size_t Ns = (size_t)1 << a.log_ns;
+    const size_t j0 = blk * W;
+    const int c = t & (W - 1);
+    const int jj = t >> LOG_W;
+    const size_t j = j0 + c;
+    const uint32_t k = (uint32_t)(j & (Ns - 1));
+
+    elem x[E];
+    // step 0 (ns2 = 1): group g holds rows jj + g*(R/E) + e*(R/E0), read from
+    // global memory with the pass twiddle applied
+#pragma unroll
+    for (int g = 0; g < G0; ++g)
+#pragma unroll
+        for (int e = 0; e < E0; ++e)
+            x[g * E0 + e] = F::load(a.in, j + (size_t)(jj + g * (R / E) + e * (R / E0)) * nR);
+    if (k != 0) {
+        const uint32_t sh = L - a.log_ns - LOG_R;
+#pragma unroll
+        for (int g = 0; g < G0; ++g)
+#pragma unroll
+            for (int e = 0; e < E0; ++e) {
+                const uint32_t r = (uint32_t)(jj + g * (R / E) + e * (R / E0));
+                if (r != 0)
+                    x[g * E0 + e] = pass_twiddle<F>(x[g * E0 + e], (r * k) << sh, a);
+            }
+    }
+    dft_groups<F, LOG_R, LOG_E0, E, G0>(x, a);
+
+    if constexpr (STEPS > 1) {
+        // step-0 outputs: column jj' lands on rows jj'*E0 + e
+#pragma unroll
+        for (int g = 0; g < G0; ++g)
+#pragma unroll
+            for (int e = 0; e < E0; ++e)
+                F::lds_store(lds, STRIDE, ((jj + g * (R / E)) * E0 + e) * W + c, x[g * E0 + e]);
+#pragma unroll
+        for (int s = 1; s < STEPS; ++s) {
+            const int log_ns2 = LOG_E0 + (s - 1) * LOG_E;
+            const int ns2 = 1 << log_ns2;
+            __syncthreads();
+            const int kk = jj & (ns2 - 1);
+#pragma unroll
+            for (int e = 1; e < E; ++e) {
+                x[e] = F::lds_load(lds, STRIDE, (jj + e * (R / E)) * W + c);
+                if (kk != 0)
+                    x[e] = F::mul(x[e], F::load_tw(a.tw_col, (size_t)((e * kk) << (LOG_R - log_ns2 - LOG_E))
+                                                                 << a.col_shift));
+            }
+            x[0] = F::lds_load(lds, STRIDE, jj * W + c);
+            dft_regs<F, LOG_R, LOG_E, 0, E>(x, a);
+            if (s < STEPS - 1) {
+                __syncthreads();
+#pragma unroll
+                for (int e = 0; e < E; ++e)
+                    F::lds_store(lds, STRIDE, ((jj >> log_ns2) * (ns2 * E) + kk + e * ns2) * W + c, x[e]);
+            }
+        }
+    }
+    // x[e] now holds output row jj + e*R/E of column c
+    if (a.flags & PASS_SCALE) {
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+            x[e] = F::mul(x[e], a.scale);
+    }
+    if (a.flags & PASS_CANON) {
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+            x[e] = F::canon(x[e]);
+    }
+    if (Ns >= (size_t)W) {
+        // the W columns share j/Ns: each output row is W contiguous elements
+        const size_t base = (j0 / Ns) * Ns * R + k;
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+            F::store(a.out, base + (size_t)(jj + e * (R / E)) * Ns, x[e]);
+    } else {
+        // Ns < W: the tile's outputs form one contiguous run of TILE
+        // elements; transpose through LDS and store it linearly
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+            F::lds_store(lds, STRIDE, (jj + e * (R / E)) * W + c, x[e]);
+        __syncthreads();
+        const size_t base = j0 * R;  // j0 is a multiple of W, hence of Ns
+        const uint32_t log_ns = a.log_ns;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int o = t + e * NT;
+            const int c_hi = o >> (log_ns + LOG_R);
+            const int rem = o & ((1 << (log_ns + LOG_R)) - 1);
+            const int r2 = rem >> log_ns;
+            const int c_lo = rem & ((1 << log_ns) - 1);
+            const int cc = (c_hi << log_ns) + c_lo;
+            F::store(a.out, base + o, F::lds_load(lds, STRIDE, r2 * W + cc));
+        }
+    }
+}
+
+template <class F, int LOG_R, int LOG_W, int LOG_E>
+__global__ __launch_bounds__(1 << (LOG_R + LOG_W - LOG_E)) void ntt_pass_kernel(PassArgs<F> a)
+{
+    pass_body<F, LOG_R, LOG_W, LOG_E>(a, blockIdx.x);
+}
+
+template <class F, int LOG_R, int LOG_W, int LOG_E>
+__global__ __launch_bounds__(1 << (LOG_R + LOG_W - LOG_E)) void ntt_pass_multi_kernel(
+    const PassJob<F> *__restrict__ jobs, const uint32_t *__restrict__ tile_job)
+{
+    const PassJob<F> &job = jobs[tile_job[blockIdx.x]];
+    pass_body<F, LOG_R, LOG_W, LOG_E>(job.a, blockIdx.x - job.first_tile);
+}
+
+// ------------------------------------------------ 753-bit, runtime tile shape
+
+// One tile of R = 2^log_r rows by W = 2^log_w columns (log_r + log_w <= 9)
+// with radix-2 steps.  Threads past R*W/2 only join the barriers.
+template <class F>
+__device__ __forceinline__ void tile753_body(const PassArgs<F> &a, int log_r, int log_w, size_t blk)
+{
+    using elem = typename F::elem_t;
+    constexpr int STRIDE = 1 << T753_LOG_TILE;
+    __shared__ uint32_t lds[STRIDE * F::LDS_WORDS_PER_ELEM];
+
+    const int t = threadIdx.x;
+    const int active = 1 << (log_r + log_w - 1);
+    const bool on = t < active;
+    const int R = 1 << log_r, W = 1 << log_w, half = R >> 1;
+    const uint32_t L = a.log_n;
+    const size_t nR = (size_t)1 << (L - log_r);
+    const size_t Ns = (size_t)1 << a.log_ns;
+    const size_t j0 = blk * (size_t)W;
+    const int c = t & (W - 1);
+    const int jj = t >> log_w;  // in [0, R/2) for active threads
+    const size_t j = j0 + c;
+    const uint32_t k = (uint32_t)(j & (Ns - 1));
+
+    elem x[2];
+    if (on) {
+        // step 0: rows jj and jj + R/2 from global memory, pass twiddles first
+        x[1] = F::load(a.in, j + (size_t)(jj + half) * nR);
+        const uint32_t sh = L - a.log_ns - (uint32_t)log_r;
+        if (k != 0)
+            x[1] = pass_twiddle<F>(x[1], ((uint32_t)(jj + half) * k) << sh, a);
+        x[0] = F::load(a.in, j + (size_t)jj * nR);
+        if (k != 0 && jj != 0)
+            x[0] = pass_twiddle<F>(x[0], ((uint32_t)jj * k) << sh, a);
+        const elem s0 = F::add(x[0], x[1]);
+        x[1] = F::sub(x[0], x[1]);
+        x[0] = s0;
+    }
+    for (int s = 1; s < log_r; ++s) {
+        // the previous step (stride 2^(s-1)) wrote column jj to rows
+        // (jj >> (s-1)) * 2^s + kk' + e * 2^(s-1), kk' = jj mod 2^(s-1)
+        const int ps = s - 1;
+        if (on) {
+            const int row = ((jj >> ps) << s) + (jj & ((1 << ps) - 1));
+            F::lds_store(lds, STRIDE, row * W + c, x[0]);
+            F::lds_store(lds, STRIDE, (row + (1 << ps)) * W + c, x[1]);
+        }
+        __syncthreads();
+        if (on) {
+            const int kk = jj & ((1 << s) - 1);
+            x[1] = F::lds_load(lds, STRIDE, (jj + half) * W + c);
+            if (kk != 0)
+                x[1] = F::mul(x[1], F::load_tw(a.tw_col, (size_t)(kk << (log_r - s - 1)) << a.col_shift));
+            x[0] = F::lds_load(lds, STRIDE, jj * W + c);
+            const elem s0 = F::add(x[0], x[1]);
+            x[1] = F::sub(x[0], x[1]);
+            x[0] = s0;
+        }
+        __syncthreads();
+    }
+    // x[e] holds output row jj + e*R/2 of column c
+    if (on) {
+        if (a.flags & PASS_SCALE) {
+            x[0] = F::mul(x[0], a.scale);
+            x[1] = F::mul(x[1], a.scale);
+        }
+        if (a.flags & PASS_CANON) {
+            x[0] = F::canon(x[0]);
+            x[1] = F::canon(x[1]);
+        }
+    }
+    if (Ns >= (size_t)W) {
+        if (on) {
+            const size_t base = (j0 / Ns) * Ns * R + k;
+            F::store(a.out, base + (size_t)jj * Ns, x[0]);
+            F::store(a.out, base + (size_t)(jj + half) * Ns, x[1]);
+        }
+    } else {
+        // one contiguous run of R*W outputs: transpose through LDS
+        if (on) {
+            F::lds_store(lds, STRIDE, jj * W + c, x[0]);
+            F::lds_store(lds, STRIDE, (jj + half) * W + c, x[1]);
+        }
+        __syncthreads();
+        if (on) {
+            const size_t base = j0 * R;
+            const uint32_t log_ns = a.log_ns;
+            for (int e = 0; e < 2; ++e) {
+                const int o = t + e * active;
+                const int c_hi = o >> (log_ns + log_r);
+                const int rem = o & ((1 << (log_ns + log_r)) - 1);
+                const int r2 = rem >> log_ns;
+                const int c_lo = rem & ((1 << log_ns) - 1);
+                F::store(a.out, base + o, F::lds_load(lds, STRIDE, r2 * W + (c_hi << log_ns) + c_lo));
+            }
+        }
+    }
+}
+
+// Single job (jobs == nullptr) or many: blockIdx -> tile_job[] -> job.
+template <class F>
+__global__ __launch_bounds__(T753_THREADS) void ntt753_kernel(PassJob<F> single, const PassJob<F> *__restrict__ jobs,
+                                                            const uint32_t *__restrict__ tile_job)
+{
+    if (jobs == nullptr) {
+        tile753_body<F>(single.a, (int)single.log_r, (int)single.log_w, blockIdx.x);
+    } else {
+        const PassJob<F> &job = jobs[tile_job[blockIdx.x]];
+        tile753_body<F>(job.a, (int)job.log_r, (int)job.log_w, blockIdx.x - job.first_tile);
+    }
+}
+
+// Full 753-bit twiddle table: out[e] = hi[e >> h] * lo[e & (2^h - 1)],
+// canonical, as 29-bit digits.  hi values are < 2p in 24 limbs, lo are
+// canonical twiddles; both carry the 2^754 Montgomery factor.
+template <class F>
+__global__ __launch_bounds__(256) void build_tw753_kernel(TW753 *out, const E753 *hi, const TW753 *lo, uint32_t h,
+                                                         size_t n)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n)
+        return;
+    const E753 v = F::canon(F::mul(hi[e >> h], lo[e & ((1u << h) - 1u)]));
+    TW753 r;
+    F::to29(r.d, v.v);
+    out[e] = r;
+}
+
+// Elementwise: out[i] = canon(in[i] * s) (used for n <= 1 and scaling).
+template <class F>
+__global__ __launch_bounds__(256) void scale_kernel(const typename F::elem_t *in, typename F::elem_t *out,
+                                                   typename F::tw_t s, size_t n, int do_mul)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    typename F::elem_t v = F::load(in, i);
+    if (do_mul)
+        v = F::mul(v, s);
+    F::store(out, i, F::canon(v));
+}
+
+}  // namespace gsn

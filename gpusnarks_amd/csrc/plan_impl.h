@@ -1,0 +1,208 @@
+// Transform plans: pass schedule, device-resident twiddle tables, workspace.
+// Shared by the plain-transform runtime and the batch scheduler.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <vector>
+
+#include "device_field.h"
+#include "gpusnarks.h"
+#include "gsn_internal.h"
+#include "gsn_launch.h"
+#include "tuning.h"
+
+namespace gsn {
+
+// ------------------------------------------------------------ fields
+
+template <int FID>
+struct FieldInfo;
+template <>
+struct FieldInfo<GSN_FIELD_FP32> {
+    using Host = fields::Fp32;
+    using Dev = F32;
+    static constexpr unsigned TWO_ADICITY = Fp32Params::TWO_ADICITY;
+    static Host root_max() { return Host::from_raw(&Fp32Params::ROOT_MONT); }
+};
+template <>
+struct FieldInfo<GSN_FIELD_MNT4753_FR> {
+    using Host = fields::Scalar;
+    using Dev = FMnt4;
+    using Params = Mnt4753FrParams;
+    static constexpr unsigned TWO_ADICITY = Params::TWO_ADICITY;
+    static Host root_max() { return Host::from_raw(Params::ROOT_MONT); }
+};
+template <>
+struct FieldInfo<GSN_FIELD_MNT6753_FR> {
+    using Host = fields::ScalarMNT6;
+    using Dev = FMnt6;
+    using Params = Mnt6753FrParams;
+    static constexpr unsigned TWO_ADICITY = Params::TWO_ADICITY;
+    static Host root_max() { return Host::from_raw(Params::ROOT_MONT); }
+};
+
+template <class H>
+H root_of_unity(H root_max, unsigned two_adicity, unsigned log_n)
+{
+    H w = root_max;
+    for (unsigned i = log_n; i < two_adicity; ++i)
+        w = w.square();
+    return w;
+}
+
+template <class H>
+bool is_canonical(const H &x)
+{
+    return !fields::detail::geq<H::LIMBS>(x.im_rep, H::mod());
+}
+
+// omega^n == 1 and omega^(n/2) == -1 (n >= 2), omega canonical
+template <class H>
+bool valid_root(const H &w, unsigned log_n)
+{
+    if (!is_canonical(w))
+        return false;
+    if (log_n == 0)
+        return w == H::one();
+    H x = w;
+    for (unsigned i = 0; i + 1 < log_n; ++i)
+        x = x.square();
+    return x == -H::one();  // then x^2 = 1 as well
+}
+
+// w * 2^754 mod p, canonical, as 29-bit digits (the device multiplier's domain)
+template <class H, class P>
+TW753 make_tw753(const H &w)
+{
+    const H rk = H::from_raw(P::RK_RAW);
+    const H v = w * rk;
+    TW753 t{};
+    for (int i = 0; i < 26; ++i) {
+        const int bit = 29 * i, wi = bit >> 5, sh = bit & 31;
+        const uint64_t lo = v.im_rep[wi];
+        const uint64_t hi = (wi + 1 < 24) ? v.im_rep[wi + 1] : 0;
+        t.d[i] = (uint32_t)(((hi << 32) | lo) >> sh) & ((1u << 29) - 1u);
+    }
+    return t;
+}
+
+// ------------------------------------------------------------ plans
+
+template <int FID>
+struct PlanImpl final : PlanBase {
+    using Info = FieldInfo<FID>;
+    using H = typename Info::Host;
+    using F = typename Info::Dev;
+    using elem = typename F::elem_t;
+    using tw = typename F::tw_t;
+    static constexpr bool IS32 = (FID == GSN_FIELD_FP32);
+
+    std::vector<PassCfg> cfg;
+    std::vector<DeviceBuffer> col_tabs;  // 32-bit: one per pass
+    DeviceBuffer full_tab;               // 753-bit: w_n^e for e < n; 32-bit (small n): pass table
+    DeviceBuffer hi_tab, lo_tab;         // 32-bit (large n): two-level pass table
+    uint32_t lo_bits = 0;
+    DeviceBuffer scratch[2];
+    tw scale{};
+
+    static tw make_tw(const H &w);
+
+    int build(const H &omega_fwd)
+    {
+        const H w = inverse ? omega_fwd.mul_inv() : omega_fwd;
+        const size_t n = (size_t)1 << log_n;
+        cfg = choose_passes(FID, log_n);
+        num_passes = (int)cfg.size();
+        for (const PassCfg &c : cfg)
+            if (!pass_supported<F>(c))
+                return set_error(GSN_ERR_INVALID_ARGUMENT, "no kernel for the pass configuration");
+        scale = make_tw(H((uint32_t)n).mul_inv());
+        if (log_n == 0)
+            return GSN_OK;
+        GSN_TRY(build_tables(w, n));
+        device_bytes = full_tab.bytes + hi_tab.bytes + lo_tab.bytes;
+        for (auto &b : col_tabs)
+            device_bytes += b.bytes;
+        return GSN_OK;
+    }
+
+    // workspace buffers an execution needs (0, 1 or 2 of n elements)
+    int workspace_buffers() const { return cfg.size() >= 3 ? 2 : (cfg.size() == 2 ? 1 : 0); }
+
+    int ensure_scratch()
+    {
+        const size_t bytes = ((size_t)1 << log_n) * sizeof(elem);
+        for (int i = 0; i < workspace_buffers(); ++i)
+            GSN_TRY(scratch[i].alloc(bytes));
+        return GSN_OK;
+    }
+
+    int upload(DeviceBuffer &buf, const void *src, size_t bytes)
+    {
+        GSN_TRY(buf.alloc(bytes));
+        return hip_status(hipMemcpy(buf.ptr, src, bytes, hipMemcpyHostToDevice), "hipMemcpy(twiddles)");
+    }
+
+    int build_tables(const H &w, size_t n);
+
+    // Pass jobs for in -> out with workspace ws[0..1]: buffer i (0 < i < P)
+    // alternates ws[0], ws[1], so no pass reads the buffer it writes.
+    void make_jobs(const void *in, void *out, void *ws0, void *ws1, std::vector<PassJob<F>> &jobs) const
+    {
+        jobs.clear();
+        const int P = (int)cfg.size();
+        uint32_t log_ns = 0;
+        void *ws[2] = {ws0, ws1};
+        for (int i = 0; i < P; ++i) {
+            const PassCfg &c = cfg[i];
+            PassJob<F> jb{};
+            PassArgs<F> &a = jb.a;
+            a.in = (const elem *)(i == 0 ? in : ws[(i - 1) & 1]);
+            a.out = (elem *)(i == P - 1 ? out : ws[i & 1]);
+            if (IS32) {
+                a.tw_col = (const tw *)col_tabs[i].ptr;
+                a.col_shift = 0;
+                a.tw_pass = (const tw *)(lo_bits ? hi_tab.ptr : full_tab.ptr);
+                a.tw_pass_lo = (const tw *)(lo_bits ? lo_tab.ptr : nullptr);
+            } else {
+                a.tw_col = (const tw *)full_tab.ptr;
+                a.col_shift = log_n - (uint32_t)c.log_r;
+                a.tw_pass = (const tw *)full_tab.ptr;
+                a.tw_pass_lo = nullptr;
+            }
+            a.lo_bits = lo_bits;
+            a.log_n = log_n;
+            a.log_ns = log_ns;
+            a.flags = (i == P - 1) ? (PASS_CANON | (inverse ? PASS_SCALE : 0u)) : 0u;
+            a.scale = scale;
+            jb.log_r = (uint32_t)c.log_r;
+            jb.log_w = (uint32_t)c.log_w;
+            jb.num_tiles = (uint32_t)((((size_t)1 << log_n) >> c.log_r) >> c.log_w);
+            jb.first_tile = 0;
+            jobs.push_back(jb);
+            log_ns += (uint32_t)c.log_r;
+        }
+    }
+
+    int execute(const void *in, void *out, hipStream_t s) override
+    {
+        GSN_TRY(take_injected_fault());
+        const size_t n = (size_t)1 << log_n;
+        if (cfg.empty()) {  // n == 1: canonical copy
+            launch_scale<F>((const elem *)in, (elem *)out, scale, n, 0, s);
+            return hip_status(hipGetLastError(), "scale_kernel launch");
+        }
+        GSN_TRY(ensure_scratch());
+        std::vector<PassJob<F>> jobs;
+        make_jobs(in, out, scratch[0].ptr, scratch[1].ptr, jobs);
+        for (size_t i = 0; i < jobs.size(); ++i) {
+            launch_pass<F>(jobs[i], cfg[i], s);
+            GSN_TRY(hip_status(hipGetLastError(), "ntt pass launch"));
+        }
+        return GSN_OK;
+    }
+};
+
+}  // namespace gsn

@@ -1,0 +1,185 @@
+/*
+ * gpusnarks-amd: C ABI of the MI355X (gfx950) finite-field NTT.
+ *
+ * This is the boundary beneath the reference's only entry point for this
+ * path,
+ *     template <typename FieldT> void best_fft(std::vector<FieldT> &a, const FieldT &omg);
+ * declared at cuda/fft_kernel.h:24-25 and defined at cuda/fft_kernel.cu:117-147
+ * (host vector in, in place, natural order in and out).  fft_kernel.h in this
+ * directory re-creates that template on top of these functions.
+ *
+ * Conventions
+ *   - Every function returns an int status: GSN_OK (0) or a gsn_status code;
+ *     nothing here exits or aborts (the reference prints and calls exit(-1),
+ *     cuda/fft_kernel.cu:137-142).  gsn_strerror() names a code and
+ *     gsn_last_error() gives the calling thread's last detailed message.
+ *   - Elements are little-endian u32 limbs in Montgomery form: 1 limb for
+ *     GSN_FIELD_FP32, 24 limbs (96 bytes, the layout of the reference's
+ *     fields::Scalar, cuda/device_field.h:75) for the 753-bit fields.
+ *   - omega is an element of the same field and representation; a NULL
+ *     omega selects the field's standard primitive n-th root of unity.
+ *     A given omega must satisfy omega^n = 1 and omega^(n/2) = -1.
+ *   - The caller owns its buffers; the library owns plans, twiddle tables,
+ *     device workspaces and streams.  `stream` arguments are hipStream_t
+ *     values passed as void* (NULL = the default stream).
+ *   - Outputs are canonical residues in [0, p), bit-identical to the CPU
+ *     restatement of the reference host FFT (test/fft_host.h) with a correct
+ *     butterfly.
+ */
+#ifndef GPUSNARKS_AMD_H
+#define GPUSNARKS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Field identifiers. */
+enum {
+    GSN_FIELD_FP32 = 0,        /* p = 3*2^30 + 1, 2-adicity 30 */
+    GSN_FIELD_MNT4753_FR = 1,  /* the reference's modulus, cuda/device_field.h:62-65; 2-adicity 15 */
+    GSN_FIELD_MNT6753_FR = 2   /* cuda/device_field.h:445-446; 2-adicity 30 */
+};
+
+/* Status codes. */
+typedef enum {
+    GSN_OK = 0,
+    GSN_ERR_INVALID_ARGUMENT = 1,
+    GSN_ERR_NOT_POWER_OF_TWO = 2,
+    GSN_ERR_SIZE_EXCEEDS_TWO_ADICITY = 3,
+    GSN_ERR_BAD_OMEGA = 4,
+    GSN_ERR_NO_DEVICE = 5,
+    GSN_ERR_DEVICE = 6,
+    GSN_ERR_OUT_OF_MEMORY = 7,
+    GSN_ERR_ADMISSION = 8,
+    GSN_ERR_NOT_READY = 9,
+    GSN_ERR_INJECTED_FAULT = 10,
+    GSN_ERR_SHUTDOWN = 11
+} gsn_status;
+
+/* --- library ---------------------------------------------------------- */
+
+/* Name of a status code (static string). */
+const char *gsn_strerror(int status);
+/* Detailed message of the calling thread's last failing call. */
+const char *gsn_last_error(void);
+/* 10000*major + 100*minor + patch */
+int gsn_version(void);
+/* Selects and initialises a HIP device (default: device 0 on first use). */
+int gsn_device_init(int device);
+/* Bytes per element of a field (4 or 96); 0 for an unknown field. */
+size_t gsn_element_bytes(int field);
+/* log2 of the largest power-of-two transform the field supports; -1 if unknown. */
+int gsn_two_adicity(int field);
+/* Writes the standard primitive 2^log_n-th root of unity (Montgomery form). */
+int gsn_root_of_unity(int field, unsigned log_n, void *omega_out);
+/* Fault injection for tests: the next `count` transform launches fail with
+ * GSN_ERR_INJECTED_FAULT before touching the device. */
+int gsn_debug_inject_faults(int count);
+
+/* --- device plumbing: memory, streams, events ----------------------------
+ * Thin wrappers over the HIP runtime this library links, for callers that
+ * have no HIP runtime of their own (or a different one). */
+int gsn_device_malloc(void **ptr, size_t bytes);
+int gsn_device_free(void *ptr);
+int gsn_device_memory(size_t *free_bytes, size_t *total_bytes);
+int gsn_device_synchronize(void);
+/* Copies; stream may be NULL.  Synchronous with respect to the host. */
+int gsn_memcpy_htod(void *d_dst, const void *h_src, size_t bytes, void *stream);
+int gsn_memcpy_dtoh(void *h_dst, const void *d_src, size_t bytes, void *stream);
+int gsn_memcpy_dtod(void *d_dst, const void *d_src, size_t bytes, void *stream);
+int gsn_stream_create(void **stream);
+int gsn_stream_destroy(void *stream);
+int gsn_stream_synchronize(void *stream);
+int gsn_event_create(void **event);
+int gsn_event_destroy(void *event);
+int gsn_event_record(void *event, void *stream);
+int gsn_event_synchronize(void *event);
+int gsn_event_elapsed_ms(float *ms, void *start, void *stop);
+
+/* --- plain transform: the best_fft replacement ---------------------------
+ * cuda/fft_kernel.cu:117-147: host vector in, transform in place,
+ * synchronous.  inverse != 0 computes the inverse transform (omega^-1 and
+ * n^-1 scaling), which the reference does not provide. */
+int gsn_fft_host(int field, void *data, size_t n, const void *omega, int inverse);
+
+/* --- plans: device-resident twiddles, asynchronous execution ------------ */
+typedef struct gsn_plan gsn_plan;
+int gsn_plan_create(gsn_plan **plan, int field, unsigned log_n, const void *omega, int inverse);
+int gsn_plan_destroy(gsn_plan *plan);
+/* Transforms d_in into d_out (device pointers; d_out may equal d_in) on
+ * `stream`.  Asynchronous; the plan's workspace is reused, so calls on one
+ * plan must not overlap in time. */
+int gsn_plan_execute(gsn_plan *plan, const void *d_in, void *d_out, void *stream);
+/* Device bytes the plan holds (twiddles + workspace). */
+size_t gsn_plan_device_bytes(const gsn_plan *plan);
+/* Number of kernel launches (passes) per execution. */
+int gsn_plan_num_passes(const gsn_plan *plan);
+
+/* --- batches: many transforms, mixed fields, sizes and directions -------- */
+typedef struct {
+    int field;
+    unsigned log_n;
+    int inverse;
+    const void *omega; /* NULL: standard root */
+    const void *d_in;
+    void *d_out;       /* may equal d_in */
+} gsn_batch_item;
+
+typedef struct gsn_batch gsn_batch;
+/* Builds a batch: plans for every (field, size, direction, omega), a
+ * schedule of work tiles and, if use_graph != 0, a captured hipGraph. */
+int gsn_batch_create(gsn_batch **batch, const gsn_batch_item *items, size_t count, int use_graph);
+int gsn_batch_launch(gsn_batch *batch, void *stream);
+int gsn_batch_destroy(gsn_batch *batch);
+/* One-shot convenience: create, launch, synchronise, destroy. */
+int gsn_batch_execute(const gsn_batch_item *items, size_t count, void *stream);
+/* Kernel launches one replay of the batch issues. */
+int gsn_batch_num_launches(const gsn_batch *batch);
+
+/* --- autotuning: per-size pass schedules -----------------------------------
+ * A schedule is num_passes triples (log_r, log_w, log_e): pass radix 2^log_r,
+ * 2^log_w columns per workgroup, register radix 2^log_e; the log_r sum to
+ * log_n.  Setting one drops cached plans.  At start-up the library reads
+ * tuning.txt beside itself (or $GSN_TUNING_FILE): lines
+ * "field log_n r,w,e r,w,e ...". */
+int gsn_tuning_set(int field, unsigned log_n, const int *shapes, int num_passes);
+int gsn_tuning_get(int field, unsigned log_n, int *shapes, int max_passes, int *num_passes);
+int gsn_tuning_clear(void);
+int gsn_tuning_load(const char *path);
+
+/* --- serving: continuous batching of NTT requests ------------------------ */
+typedef struct {
+    size_t hbm_budget_bytes;   /* admission limit on device bytes in flight (0: 1/2 of free HBM) */
+    int max_inflight;          /* max requests on the device at once (0: 4096) */
+    int num_streams;           /* device streams (0: 4) */
+    int max_batch;             /* max requests fused into one launch group (0: 256) */
+} gsn_server_config;
+
+typedef struct gsn_server gsn_server;
+int gsn_server_create(gsn_server **server, const gsn_server_config *config);
+/* Queues a request; h_in/h_out are host buffers of n elements (may alias)
+ * that must stay valid until the request completes.  Returns a ticket. */
+int gsn_server_submit(gsn_server *server, int field, unsigned log_n, int inverse, const void *omega,
+                      const void *h_in, void *h_out, uint64_t *ticket);
+/* Same with operands already resident in device memory (d_in/d_out may
+ * alias); nothing is copied. */
+int gsn_server_submit_device(gsn_server *server, int field, unsigned log_n, int inverse, const void *omega,
+                             const void *d_in, void *d_out, uint64_t *ticket);
+/* *done = 1 when the ticket has completed (its status is returned). */
+int gsn_server_poll(gsn_server *server, uint64_t ticket, int *done);
+/* Blocks until the ticket completes; returns its status. */
+int gsn_server_wait(gsn_server *server, uint64_t ticket);
+/* Arrival, start and completion times of a finished ticket (seconds on the
+ * server's monotonic clock). */
+int gsn_server_ticket_times(gsn_server *server, uint64_t ticket, double *t_submit, double *t_start,
+                            double *t_done);
+int gsn_server_destroy(gsn_server *server);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPUSNARKS_AMD_H */

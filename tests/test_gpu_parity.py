@@ -1,0 +1,122 @@
+"""GPU parity: the HIP transform against the golden DFT vectors and the CPU
+oracle (bit-exact), and size-independent properties at full sizes."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import gpusnarks_amd as g
+from gpusnarks_amd import synth
+from tests.oracle_lib import Oracle
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+NAMES = {g.FIELD_FP32: "fp32", g.FIELD_MNT4753_FR: "mnt4753_fr", g.FIELD_MNT6753_FR: "mnt6753_fr"}
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return Oracle()
+
+
+def inputs(field, log_n, seed):
+    return synth.uniform_residues(seed, 1 << log_n, g.MODULI[field], g.NLIMBS[field])
+
+
+@pytest.mark.parametrize("field", [g.FIELD_FP32, g.FIELD_MNT4753_FR, g.FIELD_MNT6753_FR])
+def test_golden_dft(field):
+    gv = np.load(os.path.join(GOLDEN, "ntt_%s.npz" % NAMES[field]))
+    sizes = sorted(int(k[1:].split("_")[0]) for k in gv.files if k.endswith("_input"))
+    for n in sizes:
+        say("golden", NAMES[field], n)
+        x, om = gv["n%d_input" % n], gv["n%d_omega" % n]
+        assert np.array_equal(g.best_fft(x, om, field), gv["n%d_forward" % n]), n
+        assert np.array_equal(g.best_ifft(x, om, field), gv["n%d_inverse" % n]), n
+
+
+@pytest.mark.parametrize("field,log_n", [
+    (g.FIELD_FP32, 4), (g.FIELD_FP32, 11), (g.FIELD_FP32, 12), (g.FIELD_FP32, 13), (g.FIELD_FP32, 16),
+    (g.FIELD_FP32, 17), (g.FIELD_FP32, 20), (g.FIELD_FP32, 21),
+    (g.FIELD_MNT6753_FR, 1), (g.FIELD_MNT6753_FR, 5), (g.FIELD_MNT6753_FR, 9), (g.FIELD_MNT6753_FR, 10),
+    (g.FIELD_MNT6753_FR, 13), (g.FIELD_MNT6753_FR, 16),
+    (g.FIELD_MNT4753_FR, 7), (g.FIELD_MNT4753_FR, 12), (g.FIELD_MNT4753_FR, 15),
+])
+def test_matches_oracle(orc, field, log_n):
+    say("oracle parity", NAMES[field], log_n)
+    x = inputs(field, log_n, 0x5EED0001 + log_n)
+    om = g.root_of_unity(field, log_n)
+    want = orc.serial_fft(field, x, om)
+    got = g.best_fft(x, om, field)
+    assert np.array_equal(got, want)
+    back = g.best_ifft(got, om, field)
+    assert np.array_equal(back, x)
+
+
+@pytest.mark.parametrize("field,log_n", [(g.FIELD_MNT6753_FR, 18), (g.FIELD_FP32, 24), (g.FIELD_FP32, 26)])
+def test_roundtrip_and_linearity_full_size(field, log_n):
+    say("full-size round trip", NAMES[field], log_n)
+    p = g.MODULI[field]
+    x = inputs(field, log_n, 0xA11CE + log_n)
+    X = g.best_fft(x, None, field)
+    assert np.array_equal(g.best_ifft(X, None, field), x)
+    # determinism: a second run gives the same bits
+    assert np.array_equal(g.best_fft(x, None, field), X)
+    # outputs are canonical
+    if field == g.FIELD_FP32:
+        assert (X[:, 0] < p).all()
+        # linearity on a few lanes: F(x + y) == F(x) + F(y)
+        y = inputs(field, log_n, 0xB0B + log_n)
+        s = ((x.astype(np.uint64) + y) % p).astype(np.uint32)
+        lhs = g.best_fft(s, None, field)
+        rhs = ((X.astype(np.uint64) + g.best_fft(y, None, field)) % p).astype(np.uint32)
+        assert np.array_equal(lhs, rhs)
+    else:
+        top = X[:, -1].astype(np.uint64) << 32 | X[:, -2]
+        ptop = p >> (32 * 22)
+        assert (top <= ptop).all()
+        # delta at 0 -> all ones (the Montgomery one everywhere)
+        one = g.root_of_unity(field, 0)
+        d = np.zeros_like(x)
+        d[0] = one
+        assert (g.best_fft(d, None, field) == one).all()
+
+
+def test_plan_api_device_buffers(orc):
+    field, log_n = g.FIELD_MNT6753_FR, 12
+    x = inputs(field, log_n, 77)
+    plan = g.Plan(field, log_n)
+    assert plan.num_passes == 2
+    d_in = g.DeviceArray.from_numpy(x)
+    d_out = g.DeviceArray(x.nbytes)
+    stream = g.Stream()
+    plan.execute(d_in.ptr, d_out.ptr, stream.handle)
+    stream.synchronize()
+    want = orc.serial_fft(field, x, g.root_of_unity(field, log_n))
+    assert np.array_equal(d_out.download(x.shape), want)
+    # in place
+    plan.execute(d_in.ptr, d_in.ptr, stream.handle)
+    stream.synchronize()
+    assert np.array_equal(d_in.download(x.shape), want)
+    plan.close()
+
+
+def test_errors_on_gpu():
+    with pytest.raises(g.GsnError) as e:
+        g.best_fft(np.zeros((1 << 16, 24), np.uint32), None, g.FIELD_MNT4753_FR)
+    assert e.value.status == 3  # 2^16 exceeds MNT4-753 Fr's 2-adicity 15
+    bad = g.root_of_unity(g.FIELD_FP32, 9)  # order 2^9, not 2^10
+    with pytest.raises(g.GsnError) as e:
+        g.best_fft(np.zeros(1 << 10, np.uint32), bad, g.FIELD_FP32)
+    assert e.value.status == 4
+    g.inject_faults(1)
+    with pytest.raises(g.GsnError) as e:
+        g.best_fft(np.zeros(16, np.uint32), None, g.FIELD_FP32)
+    assert e.value.status == 10
+    # and the next call works again
+    g.best_fft(np.zeros(16, np.uint32), None, g.FIELD_FP32)
