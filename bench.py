@@ -1,0 +1,323 @@
+#!/usr/bin/env python3
+"""Benchmark of the gfx950 NTT (one JSON line on stdout).
+
+Headline: butterflies/s of a forward NTT over the 753-bit field MNT6-753 Fr
+at n = 2^18 (24 x u32 Montgomery limbs per element, operands resident in
+HBM), one transform per step.  The same line carries:
+  roofline      the pass kernel against the integer-multiply (v_mad_u64_u32)
+                peak, with its measured average launch time;
+  cpu_baseline  the oracle's restatement of the reference host FFT
+                (test/fft_host.h, log_cpus = 10 as test/main.cpp:71) on this
+                host's cores, on a bounded sample;
+  batched       256 ragged transforms (75% 753-bit / 25% 32-bit, log2 n in
+                [10, 18], forward/inverse 50/50) as fused launches under a
+                captured hipGraph, against looping the plain plans;
+  serving       Poisson arrivals (70% 753-bit / 30% 32-bit, log2 n in
+                [12, 20]) through the continuous-batching server against
+                one-at-a-time plain dispatch: p50/p99 latency, throughput;
+  fp32          the 32-bit field at n = 2^26 against HBM bandwidth.
+
+    python bench.py [--gpus 1] [--steps 50] [--warmup 5]
+"""
+import argparse
+import json
+import os
+import random
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# MI355X figures: 256 CUs at 2.4 GHz; v_mad_u64_u32 issues at 64 lanes/clk/CU
+# (measured 34.7e12/s in a dependent-chain microbenchmark); HBM3E 8 TB/s.
+CUS, CLOCK_HZ, MAD_LANES_PER_CLK = 256, 2.4e9, 64
+MAD_PEAK = CUS * CLOCK_HZ * MAD_LANES_PER_CLK  # 3.93e13 v_mad_u64_u32 / s
+HBM_PEAK = 8.0e12
+# one 753-bit butterfly = one Montgomery product of 24-limb operands:
+# 2 * 24^2 + 24 = 1176 32x32->64 multiply-adds (add/sub not counted)
+MADS_PER_BFLY_753 = 1176
+REF_DERIVED_BFLY_S = 1.10e5  # reference README.md:46, 2^18 in 21.42 s, (n/2) log2 n / t
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def bfly(log_n):
+    return (1 << log_n) // 2 * log_n
+
+
+def inputs(field, log_n, seed):
+    import gpusnarks_amd as g
+    from gpusnarks_amd import synth
+    return synth.uniform_residues(seed, 1 << log_n, g.MODULI[field], g.NLIMBS[field])
+
+
+def time_plan(field, log_n, steps, warmup, inverse=False):
+    """Device-resident transform, timed with HIP events on its own stream."""
+    import gpusnarks_amd as g
+    x = inputs(field, log_n, 0x5EED0000 + log_n)
+    plan = g.Plan(field, log_n, inverse=inverse)
+    d_in = g.DeviceArray.from_numpy(x)
+    d_out = g.DeviceArray(x.nbytes)
+    s = g.Stream()
+    for _ in range(warmup):
+        plan.execute(d_in.ptr, d_out.ptr, s.handle)
+    s.synchronize()
+    e0, e1 = g.Event(), g.Event()
+    e0.record(s.handle)
+    for _ in range(steps):
+        plan.execute(d_in.ptr, d_out.ptr, s.handle)
+    e1.record(s.handle)
+    e1.synchronize()
+    ms = g.Event.elapsed_ms(e0, e1)
+    # spot check against a host round trip of the same input
+    out = d_out.download(x.shape)
+    ref = g.best_fft(x, None, field, inverse=inverse)
+    assert np.array_equal(out, ref), "device-resident result differs from the host path"
+    return ms / steps, plan.num_passes
+
+
+def cpu_baseline(log_n, threads):
+    """Oracle (test/fft_host.h restated) on the host: the reference
+    harness's parallel radix-2 FFT with log_cpus = 10, bounded sample."""
+    import ctypes
+    from tests.oracle_lib import Oracle
+    import gpusnarks_amd as g
+    orc = Oracle()
+    field = g.FIELD_MNT6753_FR
+    n = 1 << log_n
+    x = inputs(field, log_n, 0x5EED0000 + log_n)
+    om = g.root_of_unity(field, log_n)
+    log_cpus = 10
+    rows = n >> log_cpus
+    # phase 1 (fft_host.h:76-102): 1024 partial sums, each O(n) products;
+    # time a slice of the j range and scale
+    j_sample = max(threads, 16)
+    tmp = np.zeros((j_sample * rows, 24), np.uint32)
+    f = orc.field(field)
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    t0 = time.perf_counter()
+    orc.lib.or_parallel_phase1_sample(f, x.ctypes.data_as(u32p), n, om.ctypes.data_as(u32p), log_cpus, 0,
+                                      j_sample, tmp.ctypes.data_as(u32p), threads)
+    t_p1 = (time.perf_counter() - t0) * ((1 << log_cpus) / j_sample)
+    # phase 2 (fft_host.h:103-108): 1024 serial FFTs of n/1024, one core each
+    w_c = orc.pow(field, om[None], [1 << log_cpus])[0]
+    t0 = time.perf_counter()
+    orc.serial_fft(field, tmp[:rows], w_c)
+    t_p2 = (time.perf_counter() - t0) * (1 << log_cpus) / threads
+    total = t_p1 + t_p2
+    # and the O(n log n) serial transform on one core, in full
+    t0 = time.perf_counter()
+    orc.serial_fft(field, x, om)
+    t_serial = time.perf_counter() - t0
+    return {
+        "value": bfly(log_n) / total, "unit": "butterflies/s", "cores": threads, "kind": "port",
+        "sample": "oracle _basic_parallel_radix2_FFT_inner, MNT6-753 Fr n=2^%d, log_cpus=10: phase 1 timed "
+                  "for j in [0,%d) of 1024 on %d threads and scaled x%.0f; phase 2 timed for 1 of 1024 "
+                  "size-%d FFTs and scaled to %d threads" % (log_n, j_sample, threads, 1024 / j_sample, rows,
+                                                             threads),
+        "seconds_extrapolated": total,
+        "serial_radix2_1core_butterflies_per_s": bfly(log_n) / t_serial,
+        "serial_radix2_1core_seconds": t_serial,
+    }
+
+
+def ragged_workload(seed=0x5EED0003, count=256):
+    import gpusnarks_amd as g
+    rng = random.Random(seed)
+    items = []
+    for i in range(count):
+        field = g.FIELD_MNT6753_FR if i < count * 3 // 4 else g.FIELD_FP32
+        items.append((field, rng.randint(10, 18), rng.random() < 0.5))
+    rng.shuffle(items)
+    return items
+
+
+def bench_batched(reps=5):
+    import gpusnarks_amd as g
+    items = ragged_workload()
+    bufs = []
+    for k, (field, log_n, inv) in enumerate(items):
+        x = inputs(field, log_n, 0xBA7C4 + k)
+        bufs.append((g.DeviceArray.from_numpy(x), g.DeviceArray(x.nbytes), x))
+    total_bfly = sum(bfly(l) for _, l, _ in items)
+    s = g.Stream()
+    # plain: one plan execution per transform, looped on one stream
+    plans = {}
+    for field, log_n, inv in items:
+        if (field, log_n, inv) not in plans:
+            plans[(field, log_n, inv)] = g.Plan(field, log_n, inverse=inv)
+
+    def run_plain():
+        for (field, log_n, inv), (di, do, _) in zip(items, bufs):
+            plans[(field, log_n, inv)].execute(di.ptr, do.ptr, s.handle)
+
+    batch = g.Batch([(f, l, inv, di.ptr, do.ptr) for (f, l, inv), (di, do, _) in zip(items, bufs)], use_graph=True)
+
+    def timed(fn):
+        fn()
+        s.synchronize()
+        e0, e1 = g.Event(), g.Event()
+        e0.record(s.handle)
+        for _ in range(reps):
+            fn()
+        e1.record(s.handle)
+        e1.synchronize()
+        return g.Event.elapsed_ms(e0, e1) / reps
+
+    ms_plain = timed(run_plain)
+    outs_plain = [do.download(x.shape) for _, do, x in bufs]
+    ms_batch = timed(lambda: batch.launch(s.handle))
+    same = all(np.array_equal(do.download(x.shape), ref) for (_, do, x), ref in zip(bufs, outs_plain))
+    return {
+        "transforms": len(items), "mix": "192 x MNT6-753 Fr + 64 x Fp32, log2 n uniform in [10,18], fwd/inv 50/50",
+        "butterflies": total_bfly, "launches_batched": batch.num_launches,
+        "launches_plain": sum(plans[k].num_passes for k in items),
+        "ms_batched": ms_batch, "ms_plain_loop": ms_plain,
+        "butterflies_per_s_batched": total_bfly / (ms_batch * 1e-3),
+        "butterflies_per_s_plain_loop": total_bfly / (ms_plain * 1e-3),
+        "speedup": ms_plain / ms_batch, "bit_exact_vs_plain": bool(same),
+    }
+
+
+def bench_serving(requests=160, rates=(0.3, 0.7, 1.1), seed=0x5EED0004):
+    import gpusnarks_amd as g
+    rng = random.Random(seed)
+    reqs = []
+    for k in range(requests):
+        field = g.FIELD_MNT6753_FR if rng.random() < 0.7 else g.FIELD_FP32
+        reqs.append((field, rng.randint(12, 20), rng.random() < 0.5))
+    bufs = []
+    for k, (field, log_n, inv) in enumerate(reqs):
+        x = inputs(field, log_n, 0x5E4E + k)
+        bufs.append((g.DeviceArray.from_numpy(x), x))
+    plans = {key: g.Plan(*key[:2], inverse=key[2]) for key in set(reqs)}
+    s = g.Stream()
+    # plain capacity: back-to-back one-at-a-time service time
+    t0 = time.perf_counter()
+    for key, (d, _) in zip(reqs, bufs):
+        plans[key].execute(d.ptr, d.ptr, s.handle)
+        s.synchronize()
+    busy = time.perf_counter() - t0
+    capacity = requests / busy  # requests/s
+    total_bfly = sum(bfly(l) for _, l, _ in reqs)
+    out = {"requests_per_rate": requests, "mix": "70% MNT6-753 Fr / 30% Fp32, log2 n uniform in [12,20], fwd/inv",
+           "plain_capacity_req_per_s": capacity, "rates": []}
+    srv = g.Server(num_streams=4, max_batch=64)
+    for frac in rates:
+        lam = frac * capacity
+        arrivals = np.cumsum([rng.expovariate(lam) for _ in range(requests)])
+        # one-at-a-time plain dispatch (FIFO, each request runs alone)
+        lat_plain = []
+        t_start = time.perf_counter()
+        for a, key, (d, _) in zip(arrivals, reqs, bufs):
+            now = time.perf_counter() - t_start
+            if now < a:
+                time.sleep(a - now)
+            plans[key].execute(d.ptr, d.ptr, s.handle)
+            s.synchronize()
+            lat_plain.append(time.perf_counter() - t_start - a)
+        span_plain = time.perf_counter() - t_start
+        # served: continuous batching
+        tickets = []
+        t_start = time.perf_counter()
+        for a, (field, log_n, inv), (d, _) in zip(arrivals, reqs, bufs):
+            now = time.perf_counter() - t_start
+            if now < a:
+                time.sleep(a - now)
+            tickets.append(srv.submit_device(field, log_n, d.ptr, d.ptr, inverse=inv))
+        for t in tickets:
+            srv.wait(t)
+        span_srv = time.perf_counter() - t_start
+        lat_srv = []
+        for t in tickets:
+            ts, tb, td = srv.times(t)
+            lat_srv.append(td - ts)
+        p = lambda v, q: float(np.percentile(np.array(v) * 1e3, q))
+        out["rates"].append({
+            "offered_fraction_of_plain_capacity": frac, "arrival_rate_req_per_s": lam,
+            "served": {"p50_ms": p(lat_srv, 50), "p99_ms": p(lat_srv, 99),
+                       "throughput_req_per_s": requests / span_srv,
+                       "butterflies_per_s": total_bfly / span_srv},
+            "plain_one_at_a_time": {"p50_ms": p(lat_plain, 50), "p99_ms": p(lat_plain, 99),
+                                    "throughput_req_per_s": requests / span_plain,
+                                    "butterflies_per_s": total_bfly / span_plain},
+        })
+    srv.close()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--log-n", type=int, default=18)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-extras", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the box has one MI355X); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import gpusnarks_amd as g
+    field, log_n = g.FIELD_MNT6753_FR, args.log_n
+    log("headline: MNT6-753 Fr n=2^%d, %d steps after %d warmup" % (log_n, args.steps, args.warmup))
+    ms, passes = time_plan(field, log_n, args.steps, args.warmup)
+    value = bfly(log_n) / (ms * 1e-3)
+    ms_launch = ms / passes
+    mads_per_launch = bfly(log_n) / passes * MADS_PER_BFLY_753
+    achieved = mads_per_launch / (ms_launch * 1e-3)
+    roofline = {
+        "bound": "valu-int-mul", "unit": "Tmad/s",
+        "achieved": achieved / 1e12, "peak": MAD_PEAK / 1e12, "frac": achieved / MAD_PEAK,
+        "traffic": None,
+        "kernel": "ntt753_kernel (one Stockham pass, radix 2^%d)" % (log_n // passes),
+        "launches_per_step": passes, "avg_launch_ms": ms_launch,
+        "work_per_launch": "%d butterflies x 1176 v_mad_u64_u32" % (bfly(log_n) // passes),
+        "peak_note": "256 CU x 64 lanes/clk x 2.4 GHz; a microbenchmark measured 34.7e12/s",
+    }
+    traffic_file = os.path.join(ROOT, "profiles", "ntt753_n18_traffic.json")
+    if os.path.exists(traffic_file):
+        with open(traffic_file) as fh:
+            roofline["traffic"] = json.load(fh).get("hbm_bytes_per_launch")
+    line = {
+        "metric": "NTT butterflies/s, 753-bit n=2^18, batched vs plain; serving p99 latency",
+        "value": value, "unit": "butterflies/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": value / REF_DERIVED_BFLY_S, "dtype": "u32",
+        "data": "synthetic (SplitMix64 residues, seeded)",
+        "config": {"workload": "forward NTT, MNT6-753 Fr (753-bit, 24 x u32 Montgomery limbs), n=2^%d, "
+                               "operands resident in HBM, one transform per step" % log_n,
+                   "field": "MNT6-753 Fr", "log2_n": log_n, "elements_per_s": (1 << log_n) / (ms * 1e-3)},
+        "roofline": roofline,
+    }
+    if not args.skip_cpu:
+        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        log("cpu baseline on %d threads" % threads)
+        line["cpu_baseline"] = cpu_baseline(log_n, threads)
+    if not args.skip_extras:
+        log("fp32 n=2^26")
+        ms32, p32 = time_plan(g.FIELD_FP32, 26, 20, 3)
+        bytes_alg = 8 * (1 << 26) * ((26 + 14) // 15)  # 8 n per pass of a 2^15 on-chip tile
+        line["fp32_n26"] = {
+            "ms": ms32, "butterflies_per_s": bfly(26) / (ms32 * 1e-3), "passes": p32,
+            "algorithmic_hbm_bytes": bytes_alg, "achieved_GBps": bytes_alg / (ms32 * 1e-3) / 1e9,
+            "frac_of_8TBps": bytes_alg / (ms32 * 1e-3) / HBM_PEAK,
+            "actual_pass_bytes_GBps": 8 * (1 << 26) * p32 / (ms32 * 1e-3) / 1e9,
+        }
+        log("batched ragged workload")
+        line["batched"] = bench_batched()
+        log("serving")
+        line["serving"] = bench_serving()
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()
