@@ -20,6 +20,7 @@ HBM), one transform per step.  The same line carries:
     python bench.py [--gpus 1] [--steps 50] [--warmup 5]
 """
 import argparse
+import ctypes
 import json
 import os
 import random
@@ -260,6 +261,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-extras", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="only run the u32 copy kernel over this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the box has one MI355X); "
@@ -267,6 +270,14 @@ def main():
         sys.exit(2)
 
     import gpusnarks_amd as g
+    if args.copy_calibration_mib:
+        nbytes = args.copy_calibration_mib << 20
+        src, dst = g.DeviceArray(nbytes), g.DeviceArray(nbytes)
+        for _ in range(max(1, args.steps)):
+            g.check(g.lib().gsn_debug_copy_u32(ctypes.c_void_p(dst.ptr), ctypes.c_void_p(src.ptr), nbytes // 4, None))
+        g.synchronize()
+        log("copied %d MiB x %d" % (args.copy_calibration_mib, args.steps))
+        return
     field, log_n = g.FIELD_MNT6753_FR, args.log_n
     log("headline: MNT6-753 Fr n=2^%d, %d steps after %d warmup" % (log_n, args.steps, args.warmup))
     ms, passes = time_plan(field, log_n, args.steps, args.warmup)

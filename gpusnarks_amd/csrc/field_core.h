@@ -4,8 +4,8 @@
 // written once:
 //
 //   F32      p = 3*2^30 + 1, one u32 per element, Montgomery radix 2^32.
-//            Twiddles carry a precomputed companion w' = w*m' mod 2^32 so a
-//            product costs three 32-bit multiplies.
+//            Twiddles carry a precomputed companion w' = w * p^-1 mod 2^32,
+//            so a product is three 32-bit multiplies and three cheap ops.
 //   F753<P>  753-bit MNT fields, 24 x u32 limbs per element in memory.  The
 //            product is a finely integrated product-scanning Montgomery
 //            multiply over 26 digits of 29 bits: every column sum fits in a
@@ -36,24 +36,22 @@ struct F32 {
     using elem_t = uint32_t;
     struct tw_t {
         uint32_t w;   // twiddle, Montgomery form (radix 2^32)
-        uint32_t wq;  // w * m' mod 2^32
+        uint32_t wq;  // w * p^-1 mod 2^32
     };
     static constexpr int LIMBS = 1;
     static constexpr uint32_t P = Fp32Params::P;
 
-    // Montgomery product x * w * 2^-32 mod p for x, w in [0, p), result in [0, p).
-    // With lo = x*w mod 2^32 and m = lo*m' mod 2^32, lo + (m*p mod 2^32) is
-    // 0 when lo == 0 and 2^32 otherwise, and lo == 0 exactly when m == 0.
+    // Montgomery product x * w * 2^-32 mod p for x, w in [0, p), result in
+    // [0, p), by the subtractive form: with m = x*w*p^-1 mod 2^32, x*w - m*p
+    // is divisible by 2^32 and its low words cancel exactly, so
+    // (x*w - m*p) / 2^32 = hi(x*w) - hi(m*p), which lies in (-p, p).
     __device__ __forceinline__ static uint32_t mul(uint32_t x, tw_t t)
     {
         const uint32_t m = x * t.wq;
         const uint32_t h = __umulhi(x, t.w);
         const uint32_t q = __umulhi(m, P);
-        // u = h + q + (m != 0) < 1.75p; return u mod p without a 33-bit sum:
-        // k = p - h - c > 0, and u - p = q - k.
-        const uint32_t k = P - h - (m != 0u ? 1u : 0u);
-        const uint32_t r = q - k;
-        return q < k ? r + P : r;
+        const uint32_t r = h - q;
+        return h < q ? r + P : r;
     }
     __device__ __forceinline__ static uint32_t add(uint32_t a, uint32_t b)
     {

@@ -24,8 +24,9 @@ struct PassArgs {
     const typename F::tw_t *tw_pass_lo;
     uint32_t log_n;
     uint32_t log_ns;
-    uint32_t col_shift;
-    uint32_t lo_bits;
+    uint32_t col_shift;   // column table index = exponent << col_shift
+    uint32_t pass_shift;  // pass table exponent = (r * k) << pass_shift
+    uint32_t lo_bits;     // two-level pass table split (0: one table)
     uint32_t flags;
     typename F::tw_t scale;
 };

@@ -254,7 +254,7 @@ int DeviceBuffer::alloc(size_t n)
 template <>
 F32::tw_t PlanImpl<GSN_FIELD_FP32>::make_tw(const fields::Fp32 &w)
 {
-    return F32::tw_t{w.im_rep[0], w.im_rep[0] * Fp32Params::MPRIME};
+    return F32::tw_t{w.im_rep[0], w.im_rep[0] * Fp32Params::PINV};
 }
 
 template <>
@@ -272,47 +272,46 @@ template <>
 int PlanImpl<GSN_FIELD_FP32>::build_tables(const fields::Fp32 &w, size_t n)
 {
     using H = fields::Fp32;
-    // column tables: w_R^x for x < R, per pass
-    col_tabs.resize(cfg.size());
-    for (size_t i = 0; i < cfg.size(); ++i) {
-        const size_t R = (size_t)1 << cfg[i].log_r;
-        const H wr = w.pow(n / R);
-        std::vector<tw> t(R);
+    // table of root^x for x < count
+    auto powers = [](const H &root, size_t count) {
+        std::vector<tw> t(count);
         H acc = H::one();
-        for (size_t x = 0; x < R; ++x) {
+        for (size_t x = 0; x < count; ++x) {
             t[x] = make_tw(acc);
-            acc *= wr;
+            acc *= root;
         }
-        GSN_TRY(upload(col_tabs[i], t.data(), R * sizeof(tw)));
-    }
-    if (cfg.size() < 2)
-        return GSN_OK;
-    if (log_n <= 16) {
-        std::vector<tw> t(n);
-        H acc = H::one();
-        for (size_t e = 0; e < n; ++e) {
-            t[e] = make_tw(acc);
-            acc *= w;
+        return t;
+    };
+    const size_t P = cfg.size();
+    col_tabs.resize(P);
+    pass_tabs.resize(P);
+    pass_los.resize(P);
+    pass_lo_bits.assign(P, 0);
+    unsigned log_ns = 0;
+    for (size_t i = 0; i < P; ++i) {
+        const unsigned lr = (unsigned)cfg[i].log_r;
+        // column table: w_R^x, w_R = w^(n/R)
+        const std::vector<tw> col = powers(w.pow(n >> lr), (size_t)1 << lr);
+        GSN_TRY(upload(col_tabs[i], col.data(), col.size() * sizeof(tw)));
+        if (log_ns > 0) {
+            // pass table in units of w_{Ns R} = w^(n / (Ns R)); exponents r*k < Ns R
+            const unsigned u = log_ns + lr;
+            const H root = w.pow(n >> u);
+            if (u <= 20) {
+                const std::vector<tw> t = powers(root, (size_t)1 << u);
+                GSN_TRY(upload(pass_tabs[i], t.data(), t.size() * sizeof(tw)));
+            } else {
+                const unsigned lo = (u + 1) / 2;
+                const std::vector<tw> tl = powers(root, (size_t)1 << lo);
+                const std::vector<tw> th = powers(root.pow((uint64_t)1 << lo), (size_t)1 << (u - lo));
+                GSN_TRY(upload(pass_los[i], tl.data(), tl.size() * sizeof(tw)));
+                GSN_TRY(upload(pass_tabs[i], th.data(), th.size() * sizeof(tw)));
+                pass_lo_bits[i] = lo;
+            }
         }
-        lo_bits = 0;
-        return upload(full_tab, t.data(), n * sizeof(tw));
+        log_ns += lr;
     }
-    lo_bits = (log_n + 1) / 2;
-    const size_t nlo = (size_t)1 << lo_bits, nhi = n >> lo_bits;
-    std::vector<tw> lo(nlo), hi(nhi);
-    H acc = H::one();
-    for (size_t x = 0; x < nlo; ++x) {
-        lo[x] = make_tw(acc);
-        acc *= w;
-    }
-    const H step = acc;  // w^(2^lo_bits)
-    acc = H::one();
-    for (size_t y = 0; y < nhi; ++y) {
-        hi[y] = make_tw(acc);
-        acc *= step;
-    }
-    GSN_TRY(upload(lo_tab, lo.data(), nlo * sizeof(tw)));
-    return upload(hi_tab, hi.data(), nhi * sizeof(tw));
+    return GSN_OK;
 }
 
 template <class H, class F, class P>
@@ -761,6 +760,15 @@ int gsn_event_record(void *event, void *stream)
 int gsn_event_synchronize(void *event)
 {
     return hip_status(hipEventSynchronize((hipEvent_t)event), "hipEventSynchronize");
+}
+
+int gsn_debug_copy_u32(void *d_out, const void *d_in, size_t count, void *stream)
+{
+    if (!d_out || !d_in)
+        return set_error(GSN_ERR_INVALID_ARGUMENT, "NULL buffer");
+    GSN_TRY(ensure_device());
+    launch_scale<F32>((const uint32_t *)d_in, (uint32_t *)d_out, F32::tw_t{}, count, 0, (hipStream_t)stream);
+    return hip_status(hipGetLastError(), "copy launch");
 }
 
 int gsn_event_elapsed_ms(float *ms, void *start, void *stop)

@@ -42,6 +42,16 @@
 
 namespace gsn {
 
+// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on
+// XCD b mod 8), each with its own L2.  Renumbering blocks so that every XCD
+// gets one contiguous range of tiles puts neighbouring tiles -- which share
+// cache lines at the edges of their column groups -- under the same L2.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t grid)
+{
+    const uint32_t per = grid >> 3, rem = grid & 7u, x = b & 7u, slot = b >> 3;
+    return x * per + (x < rem ? x : rem) + slot;
+}
+
 __host__ __device__ constexpr int brev_bits(int v, int bits)
 {
     int r = 0;
@@ -54,6 +64,10 @@ template <class F>
 __device__ __forceinline__ typename F::elem_t pass_twiddle(const typename F::elem_t &x, uint32_t e,
                                                           const PassArgs<F> &a)
 {
+    // the two-level split is only ever set up for the 32-bit field
+    if constexpr (F::LIMBS > 1) {
+        return F::mul(x, F::load_tw(a.tw_pass, e));
+    }
     if (a.tw_pass_lo == nullptr)
         return F::mul(x, F::load_tw(a.tw_pass, e));
     const uint32_t hi = e >> a.lo_bits, lo = e & ((1u << a.lo_bits) - 1u);
@@ -142,7 +156,7 @@ __device__ __forceinline__ void pass_body(const PassArgs<F> &a, size_t blk)
         for (int e = 0; e < E0; ++e)
             x[g * E0 + e] = F::load(a.in, j + (size_t)(jj + g * (R / E) + e * (R / E0)) * nR);
     if (k != 0) {
-        const uint32_t sh = L - a.log_ns - LOG_R;
+        const uint32_t sh = a.pass_shift;
 #pragma unroll
         for (int g = 0; g < G0; ++g)
 #pragma unroll
@@ -227,15 +241,16 @@ __device__ __forceinline__ void pass_body(const PassArgs<F> &a, size_t blk)
 template <class F, int LOG_R, int LOG_W, int LOG_E>
 __global__ __launch_bounds__(1 << (LOG_R + LOG_W - LOG_E)) void ntt_pass_kernel(PassArgs<F> a)
 {
-    pass_body<F, LOG_R, LOG_W, LOG_E>(a, blockIdx.x);
+    pass_body<F, LOG_R, LOG_W, LOG_E>(a, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 template <class F, int LOG_R, int LOG_W, int LOG_E>
 __global__ __launch_bounds__(1 << (LOG_R + LOG_W - LOG_E)) void ntt_pass_multi_kernel(
     const PassJob<F> *__restrict__ jobs, const uint32_t *__restrict__ tile_job)
 {
-    const PassJob<F> &job = jobs[tile_job[blockIdx.x]];
-    pass_body<F, LOG_R, LOG_W, LOG_E>(job.a, blockIdx.x - job.first_tile);
+    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const PassJob<F> &job = jobs[tile_job[tile]];
+    pass_body<F, LOG_R, LOG_W, LOG_E>(job.a, tile - job.first_tile);
 }
 
 // ------------------------------------------------ 753-bit, runtime tile shape
@@ -247,7 +262,10 @@ __device__ __forceinline__ void tile753_body(const PassArgs<F> &a, int log_r, in
 {
     using elem = typename F::elem_t;
     constexpr int STRIDE = 1 << T753_LOG_TILE;
+    constexpr int TW_STRIDE = STRIDE / 2;
     __shared__ uint32_t lds[STRIDE * F::LDS_WORDS_PER_ELEM];
+    // the column's twiddles w_R^x, x < R/2, digit-major like the tile
+    __shared__ uint32_t tw_lds[26 * TW_STRIDE];
 
     const int t = threadIdx.x;
     const int active = 1 << (log_r + log_w - 1);
@@ -262,11 +280,20 @@ __device__ __forceinline__ void tile753_body(const PassArgs<F> &a, int log_r, in
     const size_t j = j0 + c;
     const uint32_t k = (uint32_t)(j & (Ns - 1));
 
+    // stage the column twiddles; the first barrier of the step loop orders
+    // these stores before any read
+    if (t < half && log_r > 1) {
+        const TW753 w = F::load_tw(a.tw_col, (size_t)t << a.col_shift);
+#pragma unroll
+        for (int d = 0; d < 26; ++d)
+            tw_lds[d * TW_STRIDE + t] = w.d[d];
+    }
+
     elem x[2];
     if (on) {
         // step 0: rows jj and jj + R/2 from global memory, pass twiddles first
         x[1] = F::load(a.in, j + (size_t)(jj + half) * nR);
-        const uint32_t sh = L - a.log_ns - (uint32_t)log_r;
+        const uint32_t sh = a.pass_shift;
         if (k != 0)
             x[1] = pass_twiddle<F>(x[1], ((uint32_t)(jj + half) * k) << sh, a);
         x[0] = F::load(a.in, j + (size_t)jj * nR);
@@ -289,8 +316,14 @@ __device__ __forceinline__ void tile753_body(const PassArgs<F> &a, int log_r, in
         if (on) {
             const int kk = jj & ((1 << s) - 1);
             x[1] = F::lds_load(lds, STRIDE, (jj + half) * W + c);
-            if (kk != 0)
-                x[1] = F::mul(x[1], F::load_tw(a.tw_col, (size_t)(kk << (log_r - s - 1)) << a.col_shift));
+            if (kk != 0) {
+                const int xi = kk << (log_r - s - 1);
+                TW753 w;
+#pragma unroll
+                for (int d = 0; d < 26; ++d)
+                    w.d[d] = tw_lds[d * TW_STRIDE + xi];
+                x[1] = F::mul(x[1], w);
+            }
             x[0] = F::lds_load(lds, STRIDE, jj * W + c);
             const elem s0 = F::add(x[0], x[1]);
             x[1] = F::sub(x[0], x[1]);
@@ -342,11 +375,12 @@ template <class F>
 __global__ __launch_bounds__(T753_THREADS) void ntt753_kernel(PassJob<F> single, const PassJob<F> *__restrict__ jobs,
                                                             const uint32_t *__restrict__ tile_job)
 {
+    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
     if (jobs == nullptr) {
-        tile753_body<F>(single.a, (int)single.log_r, (int)single.log_w, blockIdx.x);
+        tile753_body<F>(single.a, (int)single.log_r, (int)single.log_w, tile);
     } else {
-        const PassJob<F> &job = jobs[tile_job[blockIdx.x]];
-        tile753_body<F>(job.a, (int)job.log_r, (int)job.log_w, blockIdx.x - job.first_tile);
+        const PassJob<F> &job = jobs[tile_job[tile]];
+        tile753_body<F>(job.a, (int)job.log_r, (int)job.log_w, tile - job.first_tile);
     }
 }
 

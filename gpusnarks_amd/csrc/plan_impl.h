@@ -100,10 +100,11 @@ struct PlanImpl final : PlanBase {
     static constexpr bool IS32 = (FID == GSN_FIELD_FP32);
 
     std::vector<PassCfg> cfg;
-    std::vector<DeviceBuffer> col_tabs;  // 32-bit: one per pass
-    DeviceBuffer full_tab;               // 753-bit: w_n^e for e < n; 32-bit (small n): pass table
-    DeviceBuffer hi_tab, lo_tab;         // 32-bit (large n): two-level pass table
-    uint32_t lo_bits = 0;
+    DeviceBuffer full_tab;               // 753-bit: w_n^e for e < n (all twiddles)
+    // 32-bit, per pass: column table w_R^x (x < R) and pass table in units
+    // of w_{Ns R} -- direct, or hi/lo halves of a two-level split
+    std::vector<DeviceBuffer> col_tabs, pass_tabs, pass_los;
+    std::vector<uint32_t> pass_lo_bits;
     DeviceBuffer scratch[2];
     tw scale{};
 
@@ -122,9 +123,10 @@ struct PlanImpl final : PlanBase {
         if (log_n == 0)
             return GSN_OK;
         GSN_TRY(build_tables(w, n));
-        device_bytes = full_tab.bytes + hi_tab.bytes + lo_tab.bytes;
-        for (auto &b : col_tabs)
-            device_bytes += b.bytes;
+        device_bytes = full_tab.bytes;
+        for (auto *v : {&col_tabs, &pass_tabs, &pass_los})
+            for (auto &b : *v)
+                device_bytes += b.bytes;
         return GSN_OK;
     }
 
@@ -164,15 +166,18 @@ struct PlanImpl final : PlanBase {
             if (IS32) {
                 a.tw_col = (const tw *)col_tabs[i].ptr;
                 a.col_shift = 0;
-                a.tw_pass = (const tw *)(lo_bits ? hi_tab.ptr : full_tab.ptr);
-                a.tw_pass_lo = (const tw *)(lo_bits ? lo_tab.ptr : nullptr);
+                a.tw_pass = (const tw *)pass_tabs[i].ptr;
+                a.tw_pass_lo = (const tw *)pass_los[i].ptr;  // nullptr: direct table
+                a.lo_bits = pass_lo_bits[i];
+                a.pass_shift = 0;
             } else {
                 a.tw_col = (const tw *)full_tab.ptr;
                 a.col_shift = log_n - (uint32_t)c.log_r;
                 a.tw_pass = (const tw *)full_tab.ptr;
                 a.tw_pass_lo = nullptr;
+                a.lo_bits = 0;
+                a.pass_shift = log_n - log_ns - (uint32_t)c.log_r;
             }
-            a.lo_bits = lo_bits;
             a.log_n = log_n;
             a.log_ns = log_ns;
             a.flags = (i == P - 1) ? (PASS_CANON | (inverse ? PASS_SCALE : 0u)) : 0u;

@@ -98,6 +98,9 @@ int gsn_event_destroy(void *event);
 int gsn_event_record(void *event, void *stream);
 int gsn_event_synchronize(void *event);
 int gsn_event_elapsed_ms(float *ms, void *start, void *stop);
+/* Copies count u32 words with the library's elementwise kernel (one read and
+ * one write per word): a known-traffic reference for hardware counters. */
+int gsn_debug_copy_u32(void *d_out, const void *d_in, size_t count, void *stream);
 
 /* --- plain transform: the best_fft replacement ---------------------------
  * cuda/fft_kernel.cu:117-147: host vector in, transform in place,
