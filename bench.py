@@ -209,6 +209,14 @@ def bench_serving(requests=160, rates=(0.3, 0.7, 1.1), seed=0x5EED0004):
     out = {"requests_per_rate": requests, "mix": "70% MNT6-753 Fr / 30% Fp32, log2 n uniform in [12,20], fwd/inv",
            "plain_capacity_req_per_s": capacity, "rates": []}
     srv = g.Server(num_streams=4, max_batch=64)
+    # warm-up: one request per (field, size, direction) builds the plans'
+    # twiddle tables before anything is timed
+    seen = {}
+    for key, (d, _) in zip(reqs, bufs):
+        if key not in seen:
+            seen[key] = srv.submit_device(key[0], key[1], d.ptr, d.ptr, inverse=key[2])
+    for t in seen.values():
+        srv.wait(t)
     for frac in rates:
         lam = frac * capacity
         arrivals = np.cumsum([rng.expovariate(lam) for _ in range(requests)])
@@ -258,6 +266,8 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--log-n", type=int, default=18)
+    ap.add_argument("--field", choices=["mnt6", "mnt4", "fp32"], default="mnt6",
+                    help="field of the headline transform (profiling runs)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-extras", action="store_true")
@@ -278,8 +288,14 @@ def main():
         g.synchronize()
         log("copied %d MiB x %d" % (args.copy_calibration_mib, args.steps))
         return
-    field, log_n = g.FIELD_MNT6753_FR, args.log_n
-    log("headline: MNT6-753 Fr n=2^%d, %d steps after %d warmup" % (log_n, args.steps, args.warmup))
+    field = {"mnt6": g.FIELD_MNT6753_FR, "mnt4": g.FIELD_MNT4753_FR, "fp32": g.FIELD_FP32}[args.field]
+    log_n = args.log_n
+    log("headline: field %s n=2^%d, %d steps after %d warmup" % (args.field, log_n, args.steps, args.warmup))
+    if field != g.FIELD_MNT6753_FR:
+        ms, passes = time_plan(field, log_n, args.steps, args.warmup)
+        print(json.dumps({"field": args.field, "log2_n": log_n, "ms": ms, "passes": passes,
+                          "butterflies_per_s": bfly(log_n) / (ms * 1e-3)}))
+        return
     ms, passes = time_plan(field, log_n, args.steps, args.warmup)
     value = bfly(log_n) / (ms * 1e-3)
     ms_launch = ms / passes

@@ -30,6 +30,11 @@
 
 namespace gsn {
 
+// One spare LDS word after every 32: strides that are multiples of 32
+// words (row-to-row steps, transposed reads) then spread over the banks.
+__host__ __device__ constexpr int lds_pad(int i) { return i + (i >> 5); }
+__host__ __device__ constexpr int lds_padded_words(int n) { return n + (n >> 5) + 1; }
+
 // ------------------------------------------------------------------ F32
 
 struct F32 {
@@ -50,19 +55,21 @@ struct F32 {
         const uint32_t m = x * t.wq;
         const uint32_t h = __umulhi(x, t.w);
         const uint32_t q = __umulhi(m, P);
-        const uint32_t r = h - q;
-        return h < q ? r + P : r;
+        unsigned bw;
+        const uint32_t r = __builtin_subc(h, q, 0u, &bw);  // borrow: h < q
+        return bw ? r + P : r;
     }
     __device__ __forceinline__ static uint32_t add(uint32_t a, uint32_t b)
     {
-        const uint32_t k = P - b;  // a + b >= p  <=>  a >= k
-        const uint32_t r = a - k;
-        return a < k ? r + P : r;
+        unsigned bw;
+        const uint32_t r = __builtin_subc(a, P - b, 0u, &bw);  // a + b - p; borrow iff a + b < p
+        return bw ? r + P : r;
     }
     __device__ __forceinline__ static uint32_t sub(uint32_t a, uint32_t b)
     {
-        const uint32_t r = a - b;
-        return a < b ? r + P : r;
+        unsigned bw;
+        const uint32_t r = __builtin_subc(a, b, 0u, &bw);
+        return bw ? r + P : r;
     }
     __device__ __forceinline__ static uint32_t canon(uint32_t a) { return a; }
 
@@ -73,17 +80,17 @@ struct F32 {
         const uint2 v = *reinterpret_cast<const uint2 *>(tab + i);
         return tw_t{v.x, v.y};
     }
-    // LDS: plain u32 array, element i at word i.
+    // LDS: u32 array, element i at word lds_pad(i).
     static constexpr int LDS_WORDS_PER_ELEM = 1;
     __device__ __forceinline__ static void lds_store(uint32_t *lds, int stride, int i, const elem_t &v)
     {
         (void)stride;
-        lds[i] = v;
+        lds[lds_pad(i)] = v;
     }
     __device__ __forceinline__ static elem_t lds_load(const uint32_t *lds, int stride, int i)
     {
         (void)stride;
-        return lds[i];
+        return lds[lds_pad(i)];
     }
 };
 
@@ -256,21 +263,32 @@ struct F753 {
         }
         return r;
     }
-    // LDS: limb-major (structure of arrays), limb l of element i at
-    // l * stride + i, so lanes touching consecutive elements hit distinct banks.
+    // LDS: six planes of 16-byte quads; limbs 4q..4q+3 of element i sit in
+    // quad q * stride + lds_pad(i), so an element moves with six
+    // ds_write_b128 / ds_read_b128 and lanes touching consecutive elements
+    // touch consecutive quads.  `stride` counts quads per plane.
     static constexpr int LDS_WORDS_PER_ELEM = 24;
     __device__ __forceinline__ static void lds_store(uint32_t *lds, int stride, int i, const E753 &x)
     {
+        uint4 *q4 = reinterpret_cast<uint4 *>(lds);
+        const int pi = lds_pad(i);
 #pragma unroll
-        for (int l = 0; l < 24; ++l)
-            lds[l * stride + i] = x.v[l];
+        for (int q = 0; q < 6; ++q)
+            q4[q * stride + pi] = make_uint4(x.v[4 * q], x.v[4 * q + 1], x.v[4 * q + 2], x.v[4 * q + 3]);
     }
     __device__ __forceinline__ static E753 lds_load(const uint32_t *lds, int stride, int i)
     {
+        const uint4 *q4 = reinterpret_cast<const uint4 *>(lds);
+        const int pi = lds_pad(i);
         E753 r;
 #pragma unroll
-        for (int l = 0; l < 24; ++l)
-            r.v[l] = lds[l * stride + i];
+        for (int q = 0; q < 6; ++q) {
+            const uint4 v = q4[q * stride + pi];
+            r.v[4 * q] = v.x;
+            r.v[4 * q + 1] = v.y;
+            r.v[4 * q + 2] = v.z;
+            r.v[4 * q + 3] = v.w;
+        }
         return r;
     }
 };

@@ -32,8 +32,10 @@ struct Collector {
         for (size_t lvl = 0; lvl < jobs.size(); ++lvl) {
             const PassCfg &c = plan.cfg[lvl];
             // 753-bit tiles take their shape at run time: one group per level
-            const auto key = FieldInfo<FID>::Dev::LIMBS > 1 ? std::make_tuple((int)lvl, 0, 0, 1)
-                                                            : std::make_tuple((int)lvl, c.log_r, c.log_w, c.log_e);
+            // and workgroup size (tile class)
+            const auto key = FieldInfo<FID>::Dev::LIMBS > 1
+                                 ? std::make_tuple((int)lvl, t753_log_tile(c), 0, 1)
+                                 : std::make_tuple((int)lvl, c.log_r, c.log_w, c.log_e);
             by_key[key].push_back(jobs[lvl]);
         }
     }

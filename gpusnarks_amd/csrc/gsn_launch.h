@@ -47,9 +47,11 @@ struct PassCfg {
     int log_r, log_w, log_e;
 };
 
-// 753-bit tiles: at most 2^9 elements (48 KiB of LDS), 256 threads.
+// 753-bit tiles: at most 2^9 elements (48 KiB of LDS).  A tile of 2^t
+// elements runs on a workgroup of 2^(t-1) threads, t in {7, 8, 9} (tiles
+// under 2^7 use the 64-thread kernel).
 constexpr int T753_LOG_TILE = 9;
-constexpr int T753_THREADS = 256;
+constexpr int t753_log_tile(const PassCfg &c) { return c.log_r + c.log_w <= 7 ? 7 : c.log_r + c.log_w; }
 
 // Launches one pass.  For the 32-bit field the tile shape must be one of
 // the compiled instantiations (pass_supported); 753-bit shapes are runtime.

@@ -84,12 +84,13 @@ std::vector<PassCfg> default_passes(int field, unsigned L)
     if (L == 0)
         return v;
     if (field != GSN_FIELD_FP32) {
-        // 753-bit: radix-2 steps, tiles of up to 2^9 elements; spare tile
-        // capacity becomes extra columns (more threads per workgroup)
-        const int P = (int)((L + T753_LOG_TILE - 1) / T753_LOG_TILE);
+        // 753-bit: radix-2 steps in one-wave tiles of 2^7 elements (passes
+        // of at most 2^7 rows; spare tile capacity becomes extra columns)
+        const int MAX_R = 7;
+        const int P = (int)((L + MAX_R - 1) / MAX_R);
         for (int i = 0; i < P; ++i) {
             const int r = (int)L / P + (i < (int)L % P ? 1 : 0);
-            int w = T753_LOG_TILE - r;
+            int w = MAX_R - r;
             if (w > (int)L - r)
                 w = (int)L - r;
             v.push_back({r, w, 1});
@@ -294,10 +295,12 @@ int PlanImpl<GSN_FIELD_FP32>::build_tables(const fields::Fp32 &w, size_t n)
         const std::vector<tw> col = powers(w.pow(n >> lr), (size_t)1 << lr);
         GSN_TRY(upload(col_tabs[i], col.data(), col.size() * sizeof(tw)));
         if (log_ns > 0) {
-            // pass table in units of w_{Ns R} = w^(n / (Ns R)); exponents r*k < Ns R
+            // pass table in units of w_{Ns R} = w^(n / (Ns R)); exponents r*k < Ns R.
+            // Above 2^12 entries a direct table is read at scattered
+            // addresses; two tables of ~2^(u/2) entries stay in L1/L2.
             const unsigned u = log_ns + lr;
             const H root = w.pow(n >> u);
-            if (u <= 20) {
+            if (u <= 12) {
                 const std::vector<tw> t = powers(root, (size_t)1 << u);
                 GSN_TRY(upload(pass_tabs[i], t.data(), t.size() * sizeof(tw)));
             } else {

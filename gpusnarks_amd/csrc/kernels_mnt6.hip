@@ -9,21 +9,35 @@ bool pass_supported<FMnt6>(const PassCfg &c)
     return c.log_e == 1 && c.log_r >= 1 && c.log_w >= 0 && c.log_r + c.log_w <= T753_LOG_TILE;
 }
 
+static void launch753(const PassJob<FMnt6> &single, const PassJob<FMnt6> *jobs, const uint32_t *tile_job,
+                      uint32_t grid, int log_tile, hipStream_t st)
+{
+    switch (log_tile) {
+    case 7:
+        hipLaunchKernelGGL((ntt753_kernel<FMnt6, 7>), dim3(grid), dim3(64), 0, st, single, jobs, tile_job);
+        break;
+    case 8:
+        hipLaunchKernelGGL((ntt753_kernel<FMnt6, 8>), dim3(grid), dim3(128), 0, st, single, jobs, tile_job);
+        break;
+    default:
+        hipLaunchKernelGGL((ntt753_kernel<FMnt6, 9>), dim3(grid), dim3(256), 0, st, single, jobs, tile_job);
+        break;
+    }
+}
+
 template <>
 void launch_pass<FMnt6>(const PassJob<FMnt6> &job, const PassCfg &c, hipStream_t st)
 {
     const size_t grid = ((size_t)1 << (job.a.log_n - c.log_r)) >> c.log_w;
-    hipLaunchKernelGGL(ntt753_kernel<FMnt6>, dim3((unsigned)grid), dim3(T753_THREADS), 0, st, job,
-                       (const PassJob<FMnt6> *)nullptr, (const uint32_t *)nullptr);
+    launch753(job, nullptr, nullptr, (uint32_t)grid, t753_log_tile(c), st);
 }
 
 template <>
 void launch_pass_multi<FMnt6>(const PassJob<FMnt6> *jobs, const uint32_t *tile_job, uint32_t total_tiles,
                              const PassCfg &c, hipStream_t st)
 {
-    (void)c;
     PassJob<FMnt6> unused{};
-    hipLaunchKernelGGL(ntt753_kernel<FMnt6>, dim3(total_tiles), dim3(T753_THREADS), 0, st, unused, jobs, tile_job);
+    launch753(unused, jobs, tile_job, total_tiles, t753_log_tile(c), st);
 }
 
 template <>

@@ -67,16 +67,14 @@ __device__ __forceinline__ typename F::elem_t pass_twiddle(const typename F::ele
     // the two-level split is only ever set up for the 32-bit field
     if constexpr (F::LIMBS > 1) {
         return F::mul(x, F::load_tw(a.tw_pass, e));
+    } else {
+        // branch-free: entry 0 of every table is exactly 1, so a zero index
+        // multiplies by one instead of diverging the wave
+        if (a.tw_pass_lo == nullptr)
+            return F::mul(x, F::load_tw(a.tw_pass, e));
+        const uint32_t hi = e >> a.lo_bits, lo = e & ((1u << a.lo_bits) - 1u);
+        return F::mul(F::mul(x, F::load_tw(a.tw_pass, hi)), F::load_tw(a.tw_pass_lo, lo));
     }
-    if (a.tw_pass_lo == nullptr)
-        return F::mul(x, F::load_tw(a.tw_pass, e));
-    const uint32_t hi = e >> a.lo_bits, lo = e & ((1u << a.lo_bits) - 1u);
-    typename F::elem_t y = x;
-    if (hi)
-        y = F::mul(y, F::load_tw(a.tw_pass, hi));
-    if (lo)
-        y = F::mul(y, F::load_tw(a.tw_pass_lo, lo));
-    return y;
 }
 
 // Natural-order radix-2 DFT of ES = 2^LOG_ES register elements x[OFF..OFF+ES),
@@ -134,8 +132,8 @@ __device__ __forceinline__ void pass_body(const PassArgs<F> &a, size_t blk)
     constexpr int E0 = 1 << LOG_E0, G0 = E / E0;
     constexpr int TILE = R * W, NT = TILE / E;
     constexpr int STEPS = 1 + (LOG_R - LOG_E0) / LOG_E;
-    constexpr int STRIDE = TILE;  // LDS words between limbs (multi-limb fields)
-    __shared__ uint32_t lds[TILE * F::LDS_WORDS_PER_ELEM];
+    constexpr int STRIDE = lds_padded_words(TILE);  // slots per LDS plane
+    __shared__ __align__(16) uint32_t lds[STRIDE * F::LDS_WORDS_PER_ELEM];
 
     const int t = threadIdx.x;
     const uint32_t L = a.log_n;
@@ -155,14 +153,16 @@ __device__ __forceinline__ void pass_body(const PassArgs<F> &a, size_t blk)
 #pragma unroll
         for (int e = 0; e < E0; ++e)
             x[g * E0 + e] = F::load(a.in, j + (size_t)(jj + g * (R / E) + e * (R / E0)) * nR);
-    if (k != 0) {
+    // pass twiddles; the 32-bit field applies them without branches (an
+    // exponent of 0 reads the table's exact 1), the 753-bit one skips them
+    if (a.log_ns != 0 && (F::LIMBS == 1 || k != 0)) {
         const uint32_t sh = a.pass_shift;
 #pragma unroll
         for (int g = 0; g < G0; ++g)
 #pragma unroll
             for (int e = 0; e < E0; ++e) {
                 const uint32_t r = (uint32_t)(jj + g * (R / E) + e * (R / E0));
-                if (r != 0)
+                if (F::LIMBS == 1 || r != 0)
                     x[g * E0 + e] = pass_twiddle<F>(x[g * E0 + e], (r * k) << sh, a);
             }
     }
@@ -184,7 +184,7 @@ __device__ __forceinline__ void pass_body(const PassArgs<F> &a, size_t blk)
 #pragma unroll
             for (int e = 1; e < E; ++e) {
                 x[e] = F::lds_load(lds, STRIDE, (jj + e * (R / E)) * W + c);
-                if (kk != 0)
+                if (F::LIMBS == 1 || kk != 0)
                     x[e] = F::mul(x[e], F::load_tw(a.tw_col, (size_t)((e * kk) << (LOG_R - log_ns2 - LOG_E))
                                                                  << a.col_shift));
             }
@@ -257,15 +257,16 @@ __global__ __launch_bounds__(1 << (LOG_R + LOG_W - LOG_E)) void ntt_pass_multi_k
 
 // One tile of R = 2^log_r rows by W = 2^log_w columns (log_r + log_w <= 9)
 // with radix-2 steps.  Threads past R*W/2 only join the barriers.
-template <class F>
+template <class F, int LOG_TILE>
 __device__ __forceinline__ void tile753_body(const PassArgs<F> &a, int log_r, int log_w, size_t blk)
 {
     using elem = typename F::elem_t;
-    constexpr int STRIDE = 1 << T753_LOG_TILE;
-    constexpr int TW_STRIDE = STRIDE / 2;
-    __shared__ uint32_t lds[STRIDE * F::LDS_WORDS_PER_ELEM];
-    // the column's twiddles w_R^x, x < R/2, digit-major like the tile
-    __shared__ uint32_t tw_lds[26 * TW_STRIDE];
+    constexpr int STRIDE = lds_padded_words(1 << LOG_TILE);  // quads per plane
+    constexpr int TW_STRIDE = (1 << LOG_TILE) / 2;
+    __shared__ __align__(16) uint32_t lds[STRIDE * F::LDS_WORDS_PER_ELEM];
+    // the column's twiddles w_R^x, x < R/2: seven planes of digit quads
+    // (digits 26 and 27 are padding)
+    __shared__ uint4 tw_lds[7 * TW_STRIDE];
 
     const int t = threadIdx.x;
     const int active = 1 << (log_r + log_w - 1);
@@ -285,8 +286,9 @@ __device__ __forceinline__ void tile753_body(const PassArgs<F> &a, int log_r, in
     if (t < half && log_r > 1) {
         const TW753 w = F::load_tw(a.tw_col, (size_t)t << a.col_shift);
 #pragma unroll
-        for (int d = 0; d < 26; ++d)
-            tw_lds[d * TW_STRIDE + t] = w.d[d];
+        for (int q = 0; q < 7; ++q)
+            tw_lds[q * TW_STRIDE + t] = make_uint4(w.d[4 * q], w.d[4 * q + 1], q < 6 ? w.d[4 * q + 2] : 0u,
+                                                   q < 6 ? w.d[4 * q + 3] : 0u);
     }
 
     elem x[2];
@@ -320,8 +322,15 @@ __device__ __forceinline__ void tile753_body(const PassArgs<F> &a, int log_r, in
                 const int xi = kk << (log_r - s - 1);
                 TW753 w;
 #pragma unroll
-                for (int d = 0; d < 26; ++d)
-                    w.d[d] = tw_lds[d * TW_STRIDE + xi];
+                for (int q = 0; q < 7; ++q) {
+                    const uint4 v = tw_lds[q * TW_STRIDE + xi];
+                    w.d[4 * q] = v.x;
+                    w.d[4 * q + 1] = v.y;
+                    if (q < 6) {
+                        w.d[4 * q + 2] = v.z;
+                        w.d[4 * q + 3] = v.w;
+                    }
+                }
                 x[1] = F::mul(x[1], w);
             }
             x[0] = F::lds_load(lds, STRIDE, jj * W + c);
@@ -371,16 +380,20 @@ __device__ __forceinline__ void tile753_body(const PassArgs<F> &a, int log_r, in
 }
 
 // Single job (jobs == nullptr) or many: blockIdx -> tile_job[] -> job.
-template <class F>
-__global__ __launch_bounds__(T753_THREADS) void ntt753_kernel(PassJob<F> single, const PassJob<F> *__restrict__ jobs,
-                                                            const uint32_t *__restrict__ tile_job)
+// LOG_TILE sets the workgroup: 2^(LOG_TILE-1) threads for tiles of up to
+// 2^LOG_TILE elements.  A one-wave workgroup (LOG_TILE = 7) never waits on
+// another wave at its barriers.
+template <class F, int LOG_TILE>
+__global__ __launch_bounds__(1 << (LOG_TILE - 1)) void ntt753_kernel(PassJob<F> single,
+                                                                   const PassJob<F> *__restrict__ jobs,
+                                                                   const uint32_t *__restrict__ tile_job)
 {
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
     if (jobs == nullptr) {
-        tile753_body<F>(single.a, (int)single.log_r, (int)single.log_w, tile);
+        tile753_body<F, LOG_TILE>(single.a, (int)single.log_r, (int)single.log_w, tile);
     } else {
         const PassJob<F> &job = jobs[tile_job[tile]];
-        tile753_body<F>(job.a, (int)job.log_r, (int)job.log_w, tile - job.first_tile);
+        tile753_body<F, LOG_TILE>(job.a, (int)job.log_r, (int)job.log_w, tile - job.first_tile);
     }
 }
 
