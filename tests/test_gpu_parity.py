@@ -120,3 +120,16 @@ def test_errors_on_gpu():
     assert e.value.status == 10
     # and the next call works again
     g.best_fft(np.zeros(16, np.uint32), None, g.FIELD_FP32)
+
+
+def test_cpp_drop_in_gpu(tmp_path):
+    """The reference's call, best_fft<FieldT>(vector, omega), from a plain
+    C++ program linked to the library: each field against an O(n^2) DFT
+    computed with the host field type, and best_ifft back to the input."""
+    import subprocess
+    from tests.test_library import build_dropin
+    exe = build_dropin(tmp_path)
+    say("c++ drop-in (subprocess)", exe)
+    r = subprocess.run([exe, "gpu"], capture_output=True, text=True, timeout=100)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "gpu: 0 failure(s)" in r.stdout

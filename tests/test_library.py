@@ -57,3 +57,27 @@ def test_argument_errors_before_device():
     assert e.value.status == 3
     with pytest.raises(ValueError):
         g.best_fft(np.zeros((4, 3), np.uint32), None, g.FIELD_MNT6753_FR)
+
+
+def build_dropin(tmp_path):
+    """Compiles tests/cpp/dropin_main.cpp as a caller of the reference's
+    entry point would be: g++, include/, libgpusnarks.so, no HIP headers."""
+    exe = str(tmp_path / "dropin_main")
+    libdir = os.path.dirname(g.library_path)
+    cmd = ["g++", "-O1", "-std=c++17", "-I" + os.path.join(ROOT, "include"),
+           os.path.join(ROOT, "tests", "cpp", "dropin_main.cpp"), "-o", exe,
+           "-L" + libdir, "-l:" + os.path.basename(g.library_path), "-Wl,-rpath," + libdir]
+    import subprocess
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    return exe
+
+
+def test_cpp_drop_in(tmp_path):
+    """best_fft<fields::Scalar> etc. compile and link from plain C++; host
+    identities hold and bad arguments throw std::runtime_error."""
+    import subprocess
+    exe = build_dropin(tmp_path)
+    r = subprocess.run([exe, "host"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "host: 0 failure(s)" in r.stdout
