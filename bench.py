@@ -185,6 +185,47 @@ def bench_batched(reps=5):
     }
 
 
+def bench_batched_same(log_n=18, count=16, reps=5):
+    """`count` MNT6-753 Fr transforms of one size as one fused schedule
+    (hipGraph), against looping the plain plan over the same operands."""
+    import gpusnarks_amd as g
+    field = g.FIELD_MNT6753_FR
+    bufs = []
+    for k in range(count):
+        x = inputs(field, log_n, 0xB18 + k)
+        bufs.append((g.DeviceArray.from_numpy(x), g.DeviceArray(x.nbytes), x))
+    plan = g.Plan(field, log_n)
+    s = g.Stream()
+    batch = g.Batch([(field, log_n, False, di.ptr, do.ptr) for di, do, _ in bufs], use_graph=True)
+
+    def timed(fn):
+        fn()
+        s.synchronize()
+        e0, e1 = g.Event(), g.Event()
+        e0.record(s.handle)
+        for _ in range(reps):
+            fn()
+        e1.record(s.handle)
+        e1.synchronize()
+        return g.Event.elapsed_ms(e0, e1) / reps
+
+    def run_plain():
+        for di, do, _ in bufs:
+            plan.execute(di.ptr, do.ptr, s.handle)
+
+    ms_plain = timed(run_plain)
+    ref = [do.download(x.shape) for _, do, x in bufs]
+    ms_batch = timed(lambda: batch.launch(s.handle))
+    same = all(np.array_equal(do.download(x.shape), r) for (_, do, x), r in zip(bufs, ref))
+    total = count * bfly(log_n)
+    return {"transforms": count, "log2_n": log_n, "field": "MNT6-753 Fr",
+            "ms_batched": ms_batch, "ms_plain_loop": ms_plain,
+            "butterflies_per_s_batched": total / (ms_batch * 1e-3),
+            "butterflies_per_s_plain_loop": total / (ms_plain * 1e-3),
+            "roofline_frac_batched": total * MADS_PER_BFLY_753 / (ms_batch * 1e-3) / MAD_PEAK,
+            "launches_batched": batch.num_launches, "bit_exact_vs_plain": bool(same)}
+
+
 def bench_serving(requests=160, rates=(0.3, 0.7, 1.1), seed=0x5EED0004):
     import gpusnarks_amd as g
     rng = random.Random(seed)
@@ -341,6 +382,8 @@ def main():
         }
         log("batched ragged workload")
         line["batched"] = bench_batched()
+        log("batched 16 x 2^%d" % log_n)
+        line["batched_same_size"] = bench_batched_same(log_n)
         log("serving")
         line["serving"] = bench_serving()
     print(json.dumps(line))

@@ -77,13 +77,42 @@ __device__ __forceinline__ typename F::elem_t pass_twiddle(const typename F::ele
     }
 }
 
+// ------------------------------------------------ 32-bit field, compile-time tile shape
+
+// Column-table entries that every lane reads at the same index come through
+// the scalar cache: the tables are written before a launch and never during
+// it, so they can be read through the constant address space, which costs
+// no vector instructions and no VGPRs.
+__device__ __forceinline__ F32::tw_t tw32_uniform(const F32::tw_t *tab, uint32_t i)
+{
+    typedef __attribute__((address_space(4))) const unsigned long long cu64;
+    const unsigned long long v = ((cu64 *)tab)[i];
+    return F32::tw_t{(uint32_t)v, (uint32_t)(v >> 32)};
+}
+
+// Element access with a 32-bit byte offset (n <= 2^30 elements), which maps
+// to the scalar-base + 32-bit vector-offset form of the global load/store.
+__device__ __forceinline__ uint32_t ld32(const uint32_t *base, uint32_t i)
+{
+    return *reinterpret_cast<const uint32_t *>(reinterpret_cast<const char *>(base) + (i << 2));
+}
+__device__ __forceinline__ void st32(uint32_t *base, uint32_t i, uint32_t v)
+{
+    *reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(base) + (i << 2)) = v;
+}
+__device__ __forceinline__ F32::tw_t ldtw32(const F32::tw_t *tab, uint32_t i)
+{
+    const uint2 v = *reinterpret_cast<const uint2 *>(reinterpret_cast<const char *>(tab) + (i << 3));
+    return F32::tw_t{v.x, v.y};
+}
+
 // Natural-order radix-2 DFT of ES = 2^LOG_ES register elements x[OFF..OFF+ES),
 // root w_ES = w_R^(R/ES); w_{2m}^jx = w_R^(jx * R / 2m) from the column table.
-template <class F, int LOG_R, int LOG_ES, int OFF, int N>
-__device__ __forceinline__ void dft_regs(typename F::elem_t (&x)[N], const PassArgs<F> &a)
+template <int LOG_R, int LOG_ES, int OFF, int N>
+__device__ __forceinline__ void dft32_regs(uint32_t (&x)[N], const F32::tw_t *tw_col)
 {
     constexpr int ES = 1 << LOG_ES;
-    typename F::elem_t y[ES];
+    uint32_t y[ES];
 #pragma unroll
     for (int i = 0; i < ES; ++i)
         y[brev_bits(i, LOG_ES)] = x[OFF + i];
@@ -94,11 +123,11 @@ __device__ __forceinline__ void dft_regs(typename F::elem_t (&x)[N], const PassA
         for (int kb = 0; kb < ES; kb += 2 * m) {
 #pragma unroll
             for (int jx = 0; jx < m; ++jx) {
-                typename F::elem_t t = y[kb + jx + m];
+                uint32_t t = y[kb + jx + m];
                 if (jx)
-                    t = F::mul(t, F::load_tw(a.tw_col, (size_t)(jx << (LOG_R - s)) << a.col_shift));
-                y[kb + jx + m] = F::sub(y[kb + jx], t);
-                y[kb + jx] = F::add(y[kb + jx], t);
+                    t = F32::mul(t, tw32_uniform(tw_col, (uint32_t)jx << (LOG_R - s)));
+                y[kb + jx + m] = F32::sub(y[kb + jx], t);
+                y[kb + jx] = F32::add(y[kb + jx], t);
             }
         }
     }
@@ -107,66 +136,91 @@ __device__ __forceinline__ void dft_regs(typename F::elem_t (&x)[N], const PassA
         x[OFF + i] = y[i];
 }
 
-template <class F, int LOG_R, int LOG_ES, int N, int G>
-__device__ __forceinline__ void dft_groups(typename F::elem_t (&x)[N], const PassArgs<F> &a)
+template <int LOG_R, int LOG_ES, int N, int G>
+__device__ __forceinline__ void dft32_groups(uint32_t (&x)[N], const F32::tw_t *tw_col)
 {
     if constexpr (G > 0) {
-        dft_groups<F, LOG_R, LOG_ES, N, G - 1>(x, a);
-        dft_regs<F, LOG_R, LOG_ES, (G - 1) << LOG_ES, N>(x, a);
+        dft32_groups<LOG_R, LOG_ES, N, G - 1>(x, tw_col);
+        dft32_regs<LOG_R, LOG_ES, (G - 1) << LOG_ES, N>(x, tw_col);
     }
 }
-
-// ------------------------------------------------ compile-time tile shape
 
 // Column steps: step 0 has radix E0 = 2^LOG_E0 (LOG_R mod LOG_E, or LOG_E
 // when that is 0) and every later step radix E.  In a step of radix ES each
 // thread owns G = E/ES groups; group g is column index jj' = jj + g*(R/E)
-// of that step and holds rows jj' + e*(R/ES).
-template <class F, int LOG_R, int LOG_W, int LOG_E>
-__device__ __forceinline__ void pass_body(const PassArgs<F> &a, size_t blk)
+// of that step and holds rows jj' + e*(R/ES).  The twiddles between steps
+// vary per thread and are read from an LDS copy of the column table.
+//
+// LDS layout: with W >= 16 columns, row r of the tile sits at words
+// r*(W+1) .. r*(W+1)+W-1 (one pad word per row keeps the 16-lane column
+// groups of a wave on distinct banks), so every access is a per-thread base
+// plus a compile-time offset; single-column tiles use lds_pad.
+template <int LOG_R, int LOG_W, int LOG_E>
+__device__ __forceinline__ void pass32_body(const PassArgs<F32> &a, uint32_t blk)
 {
-    using elem = typename F::elem_t;
+    using F = F32;
     static_assert(LOG_E <= LOG_R, "register radix larger than the column");
     constexpr int R = 1 << LOG_R, W = 1 << LOG_W, E = 1 << LOG_E;
     constexpr int LOG_E0 = (LOG_R % LOG_E) ? (LOG_R % LOG_E) : LOG_E;
     constexpr int E0 = 1 << LOG_E0, G0 = E / E0;
     constexpr int TILE = R * W, NT = TILE / E;
     constexpr int STEPS = 1 + (LOG_R - LOG_E0) / LOG_E;
-    constexpr int STRIDE = lds_padded_words(TILE);  // slots per LDS plane
-    __shared__ __align__(16) uint32_t lds[STRIDE * F::LDS_WORDS_PER_ELEM];
+    constexpr bool ROWPAD = W >= 16;
+    constexpr int LDS_WORDS = ROWPAD ? R * (W + 1) : lds_padded_words(TILE);
+    __shared__ uint32_t lds[LDS_WORDS];
+    __shared__ F::tw_t twl[STEPS > 1 ? R : 1];
+    auto li = [](int row, int col) { return ROWPAD ? row * (W + 1) + col : lds_pad(row * W + col); };
 
     const int t = threadIdx.x;
     const uint32_t L = a.log_n;
-    const size_t nR = (size_t)1 << (L - LOG_R);
-    const size_t Ns = (size_t)1 << a.log_ns;
-    const size_t j0 = blk * W;
+    const uint32_t nR = 1u << (L - LOG_R);
+    const uint32_t ns_mask = (1u << a.log_ns) - 1u;
+    const uint32_t j0 = blk * W;
     const int c = t & (W - 1);
     const int jj = t >> LOG_W;
-    const size_t j = j0 + c;
-    const uint32_t k = (uint32_t)(j & (Ns - 1));
+    const uint32_t j = j0 + c;
+    const uint32_t k = j & ns_mask;
 
-    elem x[E];
-    // step 0 (ns2 = 1): group g holds rows jj + g*(R/E) + e*(R/E0), read from
-    // global memory with the pass twiddle applied
+    // column table into LDS; the first barrier of the step loop orders these
+    // stores before any read
+    if constexpr (STEPS > 1) {
+        for (int i = t; i < R; i += NT)
+            twl[i] = ldtw32(a.tw_col, (uint32_t)i);
+    }
+
+    uint32_t x[E];
+    // step 0: group g holds rows jj + g*(R/E) + e*(R/E0), pass twiddle applied
 #pragma unroll
     for (int g = 0; g < G0; ++g)
 #pragma unroll
         for (int e = 0; e < E0; ++e)
-            x[g * E0 + e] = F::load(a.in, j + (size_t)(jj + g * (R / E) + e * (R / E0)) * nR);
-    // pass twiddles; the 32-bit field applies them without branches (an
-    // exponent of 0 reads the table's exact 1), the 753-bit one skips them
-    if (a.log_ns != 0 && (F::LIMBS == 1 || k != 0)) {
+            x[g * E0 + e] = ld32(a.in, j + (uint32_t)(jj + g * (R / E) + e * (R / E0)) * nR);
+    if (a.log_ns != 0) {
+        // branch-free within the wave: an exponent of 0 reads the table's
+        // exact 1; the table form (direct or two-level) is uniform
         const uint32_t sh = a.pass_shift;
+        if (a.tw_pass_lo == nullptr) {
 #pragma unroll
-        for (int g = 0; g < G0; ++g)
+            for (int g = 0; g < G0; ++g)
 #pragma unroll
-            for (int e = 0; e < E0; ++e) {
-                const uint32_t r = (uint32_t)(jj + g * (R / E) + e * (R / E0));
-                if (F::LIMBS == 1 || r != 0)
-                    x[g * E0 + e] = pass_twiddle<F>(x[g * E0 + e], (r * k) << sh, a);
-            }
+                for (int e = 0; e < E0; ++e) {
+                    const uint32_t r = (uint32_t)(jj + g * (R / E) + e * (R / E0));
+                    x[g * E0 + e] = F::mul(x[g * E0 + e], ldtw32(a.tw_pass, (r * k) << sh));
+                }
+        } else {
+            const uint32_t lb = a.lo_bits, lm = (1u << lb) - 1u;
+#pragma unroll
+            for (int g = 0; g < G0; ++g)
+#pragma unroll
+                for (int e = 0; e < E0; ++e) {
+                    const uint32_t r = (uint32_t)(jj + g * (R / E) + e * (R / E0));
+                    const uint32_t ex = (r * k) << sh;
+                    x[g * E0 + e] = F::mul(F::mul(x[g * E0 + e], ldtw32(a.tw_pass, ex >> lb)),
+                                           ldtw32(a.tw_pass_lo, ex & lm));
+                }
+        }
     }
-    dft_groups<F, LOG_R, LOG_E0, E, G0>(x, a);
+    dft32_groups<LOG_R, LOG_E0, E, G0>(x, a.tw_col);
 
     if constexpr (STEPS > 1) {
         // step-0 outputs: column jj' lands on rows jj'*E0 + e
@@ -174,7 +228,7 @@ __device__ __forceinline__ void pass_body(const PassArgs<F> &a, size_t blk)
         for (int g = 0; g < G0; ++g)
 #pragma unroll
             for (int e = 0; e < E0; ++e)
-                F::lds_store(lds, STRIDE, ((jj + g * (R / E)) * E0 + e) * W + c, x[g * E0 + e]);
+                lds[li((jj + g * (R / E)) * E0 + e, c)] = x[g * E0 + e];
 #pragma unroll
         for (int s = 1; s < STEPS; ++s) {
             const int log_ns2 = LOG_E0 + (s - 1) * LOG_E;
@@ -182,19 +236,16 @@ __device__ __forceinline__ void pass_body(const PassArgs<F> &a, size_t blk)
             __syncthreads();
             const int kk = jj & (ns2 - 1);
 #pragma unroll
-            for (int e = 1; e < E; ++e) {
-                x[e] = F::lds_load(lds, STRIDE, (jj + e * (R / E)) * W + c);
-                if (F::LIMBS == 1 || kk != 0)
-                    x[e] = F::mul(x[e], F::load_tw(a.tw_col, (size_t)((e * kk) << (LOG_R - log_ns2 - LOG_E))
-                                                                 << a.col_shift));
-            }
-            x[0] = F::lds_load(lds, STRIDE, jj * W + c);
-            dft_regs<F, LOG_R, LOG_E, 0, E>(x, a);
+            for (int e = 1; e < E; ++e)
+                x[e] = F::mul(lds[li(jj + e * (R / E), c)],
+                              twl[(e * kk) << (LOG_R - log_ns2 - LOG_E)]);
+            x[0] = lds[li(jj, c)];
+            dft32_regs<LOG_R, LOG_E, 0, E>(x, a.tw_col);
             if (s < STEPS - 1) {
                 __syncthreads();
 #pragma unroll
                 for (int e = 0; e < E; ++e)
-                    F::lds_store(lds, STRIDE, ((jj >> log_ns2) * (ns2 * E) + kk + e * ns2) * W + c, x[e]);
+                    lds[li((jj >> log_ns2) * (ns2 * E) + kk + e * ns2, c)] = x[e];
             }
         }
     }
@@ -204,26 +255,22 @@ __device__ __forceinline__ void pass_body(const PassArgs<F> &a, size_t blk)
         for (int e = 0; e < E; ++e)
             x[e] = F::mul(x[e], a.scale);
     }
-    if (a.flags & PASS_CANON) {
-#pragma unroll
-        for (int e = 0; e < E; ++e)
-            x[e] = F::canon(x[e]);
-    }
-    if (Ns >= (size_t)W) {
+    const uint32_t Ns = 1u << a.log_ns;
+    if (Ns >= (uint32_t)W) {
         // the W columns share j/Ns: each output row is W contiguous elements
-        const size_t base = (j0 / Ns) * Ns * R + k;
+        const uint32_t base = (j0 >> a.log_ns) * Ns * R + k;
 #pragma unroll
         for (int e = 0; e < E; ++e)
-            F::store(a.out, base + (size_t)(jj + e * (R / E)) * Ns, x[e]);
+            st32(a.out, base + (uint32_t)(jj + e * (R / E)) * Ns, x[e]);
     } else {
         // Ns < W: the tile's outputs form one contiguous run of TILE
         // elements; transpose through LDS and store it linearly
         __syncthreads();
 #pragma unroll
         for (int e = 0; e < E; ++e)
-            F::lds_store(lds, STRIDE, (jj + e * (R / E)) * W + c, x[e]);
+            lds[li(jj + e * (R / E), c)] = x[e];
         __syncthreads();
-        const size_t base = j0 * R;  // j0 is a multiple of W, hence of Ns
+        const uint32_t base = j0 * R;  // j0 is a multiple of W, hence of Ns
         const uint32_t log_ns = a.log_ns;
 #pragma unroll
         for (int e = 0; e < E; ++e) {
@@ -233,7 +280,7 @@ __device__ __forceinline__ void pass_body(const PassArgs<F> &a, size_t blk)
             const int r2 = rem >> log_ns;
             const int c_lo = rem & ((1 << log_ns) - 1);
             const int cc = (c_hi << log_ns) + c_lo;
-            F::store(a.out, base + o, F::lds_load(lds, STRIDE, r2 * W + cc));
+            st32(a.out, base + o, lds[li(r2, cc)]);
         }
     }
 }
@@ -241,7 +288,7 @@ __device__ __forceinline__ void pass_body(const PassArgs<F> &a, size_t blk)
 template <class F, int LOG_R, int LOG_W, int LOG_E>
 __global__ __launch_bounds__(1 << (LOG_R + LOG_W - LOG_E)) void ntt_pass_kernel(PassArgs<F> a)
 {
-    pass_body<F, LOG_R, LOG_W, LOG_E>(a, xcd_tile(blockIdx.x, gridDim.x));
+    pass32_body<LOG_R, LOG_W, LOG_E>(a, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 template <class F, int LOG_R, int LOG_W, int LOG_E>
@@ -250,7 +297,7 @@ __global__ __launch_bounds__(1 << (LOG_R + LOG_W - LOG_E)) void ntt_pass_multi_k
 {
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const PassJob<F> &job = jobs[tile_job[tile]];
-    pass_body<F, LOG_R, LOG_W, LOG_E>(job.a, tile - job.first_tile);
+    pass32_body<LOG_R, LOG_W, LOG_E>(job.a, tile - job.first_tile);
 }
 
 // ------------------------------------------------ 753-bit, runtime tile shape
@@ -351,7 +398,32 @@ __device__ __forceinline__ void tile753_body(const PassArgs<F> &a, int log_r, in
             x[1] = F::canon(x[1]);
         }
     }
-    if (Ns >= (size_t)W) {
+    if (a.log_ns == 0) {
+        // first pass: the tile's R*W outputs are one contiguous run
+        // (out[j*R + r]); stage it in LDS and write it as consecutive
+        // 16-byte quads, lane after lane, with non-temporal stores (the next
+        // pass reads it from HBM, not from this CU's caches).  The step
+        // loop ended on a barrier, so LDS is free.
+        if (on) {
+            F::lds_store(lds, STRIDE, jj * W + c, x[0]);
+            F::lds_store(lds, STRIDE, (jj + half) * W + c, x[1]);
+        }
+        __syncthreads();
+        typedef uint32_t u4v __attribute__((ext_vector_type(4)));
+        const uint4 *q4 = reinterpret_cast<const uint4 *>(lds);
+        u4v *o = reinterpret_cast<u4v *>(a.out + j0 * R);
+        const int nq = (R * W) * 6;
+        for (int m = t; m < nq; m += (int)blockDim.x) {
+            const int e = m / 6, q = m - 6 * e;
+            const uint4 v = q4[q * STRIDE + lds_pad((e & (R - 1)) * W + (e >> log_r))];
+            u4v w;
+            w.x = v.x;
+            w.y = v.y;
+            w.z = v.z;
+            w.w = v.w;
+            __builtin_nontemporal_store(w, o + m);
+        }
+    } else if (Ns >= (size_t)W) {
         if (on) {
             const size_t base = (j0 / Ns) * Ns * R + k;
             F::store(a.out, base + (size_t)jj * Ns, x[0]);
