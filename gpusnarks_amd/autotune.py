@@ -52,7 +52,7 @@ def candidates(field, log_n):
         return out
     for P in (1, 2, 3, 4):
         for comp in compositions(log_n, P, 3 if P > 1 else 1, 9):
-            for tile in (7, 8, 9):
+            for tile in (7, 8, 9, 10):
                 if max(comp) > tile:
                     continue
                 sched = [(r, min(tile - r, log_n - r), 1) for r in comp]

@@ -7,20 +7,18 @@
 //            Twiddles carry a precomputed companion w' = w * p^-1 mod 2^32,
 //            so a product is three 32-bit multiplies and three cheap ops.
 //   F753<P>  753-bit MNT fields, 24 x u32 limbs per element in memory.  The
-//            product is a finely integrated product-scanning Montgomery
-//            multiply over 26 digits of 29 bits: every column sum fits in a
-//            64-bit accumulator, so each partial product is a single
-//            v_mad_u64_u32 with no carry handling.  Its radix is 2^754.
+//            product by a twiddle estimates its quotient from a precomputed
+//            wq = floor(w 2^783 / p) (Shoup's method carried to 27 digits of
+//            29 bits): every column sum fits a 64-bit accumulator, so each
+//            partial product is a single v_mad_u64_u32 / v_mad_i64_i32 with
+//            no carry handling, 1158 of them against the 1326 of a
+//            26-digit Montgomery product.  Sums and differences are not
+//            reduced at all; the last pass reduces to [0, p).
 //
 // Replaces the reference's field core (cuda/device_field.h:67-215,
 // cuda/device_field_operators.h:92-258); the reference's 24-limb CIOS
 // (device_field_operators.h:151-187) is the shape of the oracle's multiply,
 // not of this one.
-//
-// Value ranges (753-bit): data values live in [0, 2p) between operations.
-// mul(y, w) needs y < 2p and a canonical twiddle w < p, and returns < 2p
-// because (2p * p + 2^754 * p) / 2^754 < 2p when p < 0.45 * 2^754.
-// add/sub return values in [0, 2p).  canon() maps [0, 2p) to [0, p).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -99,135 +97,217 @@ struct F32 {
 struct E753 {
     uint32_t v[24];
 };
+// A twiddle of the 753-bit fields: the residue w itself (canonical, not in
+// Montgomery form) as 26 digits of 29 bits, and wq = floor(w 2^783 / p) as
+// 27 digits; padded to 224 bytes so it moves as 14 aligned quads.
 struct TW753 {
-    uint32_t d[26];  // 29-bit digits, value w * 2^754 mod p, canonical
+    uint32_t w[26];
+    uint32_t wq[27];
+    uint32_t pad[3];
 };
 
+// Digits [29i, 29i + 29) of a value held in 24 u32 limbs, 27 digits.
+__host__ __device__ inline void limbs_to_digits27(uint32_t d[27], const uint32_t y[24])
+{
+    for (int i = 0; i < 27; ++i) {
+        const int bit = 29 * i, w = bit >> 5, sh = bit & 31;
+        const uint64_t lo = y[w];
+        const uint64_t hi = (w + 1 < 24) ? y[w + 1] : 0u;
+        d[i] = (uint32_t)(((hi << 32) | lo) >> sh) & ((1u << 29) - 1u);
+    }
+}
+
+// wq = floor(w 2^783 / p) from rem = w 2^783 mod p (canonical): w 2^783 -
+// rem is an exact multiple of p, so wq = (2^783 - rem) * p^-1 mod 2^783
+// (and wq < 2^783 since w < p).  29-bit digits throughout.
+template <class P>
+__host__ __device__ inline void shoup_wq(uint32_t wq[27], const uint32_t rem[24])
+{
+    constexpr uint32_t M = (1u << 29) - 1u;
+    uint32_t r[27], neg[27];
+    limbs_to_digits27(r, rem);
+    int64_t b = 0;  // (0 - rem) mod 2^783, digit by digit
+    for (int i = 0; i < 27; ++i) {
+        const int64_t v = -(int64_t)r[i] - b;
+        neg[i] = (uint32_t)v & M;
+        b = v < 0 ? 1 : 0;
+    }
+    uint64_t acc = 0;
+    for (int c = 0; c < 27; ++c) {
+        for (int i = 0; i <= c; ++i)
+            acc += (uint64_t)neg[i] * P::PINV783[c - i];
+        wq[c] = (uint32_t)acc & M;
+        acc >>= 29;
+    }
+}
+
+// 753-bit field arithmetic for the NTT kernels.
+//
+// Elements are 24 x u32 little-endian limbs, the public Montgomery
+// representation.  The transform is linear, so multiplying a Montgomery
+// representation by the plain twiddle value w keeps it one.  Inside a
+// transform values are only partly reduced: every intermediate is below
+// 2^767 (a transform of 2^L adds at most 3p per stage to canonical inputs,
+// and (1 + 3*30) p < 2^761), and the last pass reduces to [0, p).
+//
+// The product y*w mod p, for any y < 2^768, is formed from a quotient
+// estimate (Shoup's method with a precomputed wq = floor(w 2^783 / p)):
+//   q = floor(y * wq / 2^783)  from the top columns of y*wq only,
+//   t = y*w - q*p              computed modulo 2^783 (its low 27 digits).
+// wq >= w 2^783/p - 1 and y < 2^783 make the exact quotient estimate at most
+// one below floor(y w / p); dropping the columns below 25 of y*wq changes
+// the sum by less than 25 * 2^(58 + 29*24) < 2^759 = 2^-24 * 2^783, at most
+// one more.  So t lies in [0, 3p).  Digits are 29 bits: every column of at
+// most 27 + 27 products of 58 bits fits one 64-bit accumulator, signed for
+// the q*p terms.
 template <class P>
 struct F753 {
     using elem_t = E753;
     using tw_t = TW753;
     static constexpr int LIMBS = 24;
     static constexpr uint32_t M29 = (1u << 29) - 1u;
-    static constexpr bool MONT_FRIENDLY = (P::MOD29[0] == 1u);  // p == 1 mod 2^29
 
-    // bits [29i, 29i + 29) of a value held in 24 u32 limbs (value < 2^754)
-    __device__ __forceinline__ static void to29(uint32_t a[26], const uint32_t y[24])
+    // bits [29i, 29i + 29) of a 768-bit value (digit 26 has 14 bits)
+    __device__ __forceinline__ static void to29(uint32_t a[27], const uint32_t y[24])
     {
 #pragma unroll
-        for (int i = 0; i < 26; ++i) {
+        for (int i = 0; i < 27; ++i) {
             const int bit = 29 * i, w = bit >> 5, sh = bit & 31;
             const uint32_t lo = y[w];
             const uint32_t hi = (w + 1 < 24) ? y[w + 1] : 0u;
             // 32-bit funnel shift (v_alignbit_b32); a 64-bit shift of the
             // pair would be folded into unaligned loads of a stack copy
             const uint32_t v = sh ? ((lo >> sh) | (hi << (32 - sh))) : lo;
-            a[i] = v & M29;
+            a[i] = (i == 26) ? v : (v & M29);
         }
     }
     // inverse of to29 for digits < 2^29 and a value < 2^768
-    __device__ __forceinline__ static void from29(uint32_t r[24], const uint32_t d[26])
+    __device__ __forceinline__ static void from29(uint32_t r[24], const uint32_t d[27])
     {
 #pragma unroll
         for (int j = 0; j < 24; ++j) {
             const int bit = 32 * j, i0 = bit / 29, sh = bit % 29;
             uint32_t v = d[i0] >> sh;
-            if (i0 + 1 < 26)
+            if (i0 + 1 < 27)
                 v |= d[i0 + 1] << (29 - sh);
-            if (58 - sh < 32 && i0 + 2 < 26)
+            if (58 - sh < 32 && i0 + 2 < 27)
                 v |= d[i0 + 2] << (58 - sh);
             r[j] = v;
         }
     }
 
-    // Montgomery product y * w * 2^-754 mod p, y < 2p, w canonical in 29-bit
-    // digits; result < 2p in 24 limbs.  Finely integrated product scanning.
-    __device__ __forceinline__ static E753 mul(const E753 &y, const TW753 &w)
+    // y * w mod p as a value in [0, 3p), for any y < 2^768
+    __device__ __forceinline__ static E753 mul(const E753 &y, const TW753 &t)
     {
-        uint32_t a[26], m[26];
+        uint32_t a[27];
         to29(a, y.v);
+        // q = floor(a * wq / 2^783): columns 25..52, digits from column 27
+        uint32_t q[27];
         uint64_t acc = 0;
 #pragma unroll
-        for (int k = 0; k < 26; ++k) {
+        for (int c = 25; c <= 52; ++c) {
 #pragma unroll
-            for (int i = 0; i < k; ++i) {
-                acc += (uint64_t)a[i] * w.d[k - i];
-                acc += (uint64_t)m[i] * P::MOD29[k - i];
-            }
-            acc += (uint64_t)a[k] * w.d[0];
-            if constexpr (MONT_FRIENDLY) {
-                // m' = -1 mod 2^29 and p_0 = 1
-                m[k] = (0u - (uint32_t)acc) & M29;
-                acc += m[k];
-            } else {
-                m[k] = ((uint32_t)acc * P::MPRIME29) & M29;
-                acc += (uint64_t)m[k] * P::MOD29[0];
-            }
+            for (int i = (c > 26 ? c - 26 : 0); i <= (c < 26 ? c : 26); ++i)
+                acc += (uint64_t)a[i] * t.wq[c - i];
+            if (c >= 27)
+                q[c - 27] = (uint32_t)acc & M29;
             acc >>= 29;
         }
-        // high columns; digit k-26 of the result replaces m[k-26], which
-        // column k no longer reads
+        q[26] = (uint32_t)acc;
+        // a*w - q*p modulo 2^783: columns 0..26
+        uint32_t r[27];
+        int64_t s = 0;
 #pragma unroll
-        for (int k = 26; k < 51; ++k) {
+        for (int c = 0; c <= 26; ++c) {
 #pragma unroll
-            for (int i = k - 25; i < 26; ++i) {
-                acc += (uint64_t)a[i] * w.d[k - i];
-                acc += (uint64_t)m[i] * P::MOD29[k - i];
-            }
-            m[k - 26] = (uint32_t)acc & M29;
-            acc >>= 29;
+            for (int i = (c > 25 ? c - 25 : 0); i <= (c < 26 ? c : 26); ++i)
+                s += (int64_t)((uint64_t)a[i] * t.w[c - i]);
+#pragma unroll
+            for (int i = (c > 25 ? c - 25 : 0); i <= (c < 26 ? c : 26); ++i)
+                s += (int64_t)(int32_t)q[i] * (int64_t)(-(int32_t)P::MOD29[c - i]);
+            r[c] = (uint32_t)s & M29;
+            s >>= 29;
         }
-        m[25] = (uint32_t)acc;
-        E753 r;
-        from29(r.v, m);
-        return r;
+        E753 o;
+        from29(o.v, r);
+        return o;
     }
 
-    // (a + b) reduced to [0, 2p) for a, b in [0, 2p)
+    // a + b (no reduction: callers keep the sum below 2^768)
     __device__ __forceinline__ static E753 add(const E753 &a, const E753 &b)
     {
-        uint32_t s[24], d[24];
-        unsigned c = 0, bw = 0;
-#pragma unroll
-        for (int i = 0; i < 24; ++i)
-            s[i] = __builtin_addc(a.v[i], b.v[i], c, &c);
-#pragma unroll
-        for (int i = 0; i < 24; ++i)
-            d[i] = __builtin_subc(s[i], P::TWO_P[i], bw, &bw);
         E753 r;
+        unsigned c = 0;
 #pragma unroll
         for (int i = 0; i < 24; ++i)
-            r.v[i] = bw ? s[i] : d[i];
+            r.v[i] = __builtin_addc(a.v[i], b.v[i], c, &c);
         return r;
     }
-    // (a - b) reduced to [0, 2p) for a, b in [0, 2p)
-    __device__ __forceinline__ static E753 sub(const E753 &a, const E753 &b)
+    // a + 3p - t for t < 3p (no reduction)
+    __device__ __forceinline__ static E753 sub(const E753 &a, const E753 &t)
     {
         uint32_t d[24];
         unsigned bw = 0, c = 0;
 #pragma unroll
         for (int i = 0; i < 24; ++i)
-            d[i] = __builtin_subc(a.v[i], b.v[i], bw, &bw);
-        const uint32_t mask = 0u - (uint32_t)bw;
+            d[i] = __builtin_subc(P::THREE_P[i], t.v[i], bw, &bw);
         E753 r;
 #pragma unroll
         for (int i = 0; i < 24; ++i)
-            r.v[i] = __builtin_addc(d[i], P::TWO_P[i] & mask, c, &c);
+            r.v[i] = __builtin_addc(a.v[i], d[i], c, &c);
         return r;
     }
-    // [0, 2p) -> [0, p)
-    __device__ __forceinline__ static E753 canon(const E753 &a)
+    // x mod p for x < 2^767: a quotient estimate from the top 96 bits in
+    // double precision, q_est in {q - 1, q}, then one conditional subtraction
+    __device__ __forceinline__ static E753 canon(const E753 &x)
     {
-        uint32_t d[24];
+        constexpr double TWO32 = 4294967296.0;
+        constexpr double P_TOP = ((double)P::MOD[23] * TWO32 + (double)P::MOD[22]) * TWO32 + (double)P::MOD[21];
+        const double x_top = ((double)x.v[23] * TWO32 + (double)x.v[22]) * TWO32 + (double)x.v[21];
+        // x_top / P_TOP is x/p to within a relative 2^-50; x/p < 2^15
+        const uint32_t q = (uint32_t)(x_top * ((1.0 - 0x1p-40) / P_TOP));
+        uint32_t r[24];
+        uint64_t carry = 0;
         unsigned bw = 0;
 #pragma unroll
-        for (int i = 0; i < 24; ++i)
-            d[i] = __builtin_subc(a.v[i], P::MOD[i], bw, &bw);
-        E753 r;
+        for (int i = 0; i < 24; ++i) {
+            const uint64_t pr = (uint64_t)q * P::MOD[i] + carry;
+            carry = pr >> 32;
+            r[i] = __builtin_subc(x.v[i], (uint32_t)pr, bw, &bw);
+        }
+        uint32_t d[24];
+        bw = 0;
 #pragma unroll
         for (int i = 0; i < 24; ++i)
-            r.v[i] = bw ? a.v[i] : d[i];
-        return r;
+            d[i] = __builtin_subc(r[i], P::MOD[i], bw, &bw);
+        E753 o;
+#pragma unroll
+        for (int i = 0; i < 24; ++i)
+            o.v[i] = bw ? r[i] : d[i];
+        return o;
+    }
+
+    // the twiddle record of a canonical residue v: its digits, and wq from
+    // rem = v 2^783 mod p, itself the product of v and the constant 2^783 mod p
+    __device__ __forceinline__ static TW753 make_tw(const E753 &v)
+    {
+        TW753 c;
+#pragma unroll
+        for (int i = 0; i < 26; ++i)
+            c.w[i] = P::C783_29[i];
+#pragma unroll
+        for (int i = 0; i < 27; ++i)
+            c.wq[i] = P::C783_WQ[i];
+        const E753 rem = canon(mul(v, c));
+        TW753 o;
+        uint32_t d[27];
+        limbs_to_digits27(d, v.v);
+#pragma unroll
+        for (int i = 0; i < 26; ++i)
+            o.w[i] = d[i];
+        shoup_wq<P>(o.wq, rem.v);
+        o.pad[0] = o.pad[1] = o.pad[2] = 0;
+        return o;
     }
 
     __device__ __forceinline__ static E753 load(const E753 *base, size_t i)
@@ -253,14 +333,26 @@ struct F753 {
     }
     __device__ __forceinline__ static TW753 load_tw(const TW753 *tab, size_t i)
     {
-        const uint2 *p = reinterpret_cast<const uint2 *>(tab + i);
+        const uint4 *p = reinterpret_cast<const uint4 *>(tab + i);
+        uint32_t d[56];
+#pragma unroll
+        for (int q = 0; q < 14; ++q) {
+            const uint4 v = p[q];
+            d[4 * q] = v.x;
+            d[4 * q + 1] = v.y;
+            d[4 * q + 2] = v.z;
+            d[4 * q + 3] = v.w;
+        }
         TW753 r;
 #pragma unroll
-        for (int q = 0; q < 13; ++q) {
-            const uint2 v = p[q];
-            r.d[2 * q] = v.x;
-            r.d[2 * q + 1] = v.y;
-        }
+        for (int k = 0; k < 26; ++k)
+            r.w[k] = d[k];
+#pragma unroll
+        for (int k = 0; k < 27; ++k)
+            r.wq[k] = d[26 + k];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            r.pad[k] = 0;
         return r;
     }
     // LDS: six planes of 16-byte quads; limbs 4q..4q+3 of element i sit in

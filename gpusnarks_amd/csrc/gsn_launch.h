@@ -47,10 +47,10 @@ struct PassCfg {
     int log_r, log_w, log_e;
 };
 
-// 753-bit tiles: at most 2^9 elements (48 KiB of LDS).  A tile of 2^t
-// elements runs on a workgroup of 2^(t-1) threads, t in {7, 8, 9} (tiles
-// under 2^7 use the 64-thread kernel).
-constexpr int T753_LOG_TILE = 9;
+// 753-bit tiles: at most 2^10 elements (99 KiB of LDS, plus up to 56 KiB of
+// staged twiddles).  A tile of 2^t elements runs on a workgroup of 2^(t-1)
+// threads, t in {7, 8, 9, 10} (tiles under 2^7 use the 64-thread kernel).
+constexpr int T753_LOG_TILE = 10;
 constexpr int t753_log_tile(const PassCfg &c) { return c.log_r + c.log_w <= 7 ? 7 : c.log_r + c.log_w; }
 
 // Launches one pass.  For the 32-bit field the tile shape must be one of

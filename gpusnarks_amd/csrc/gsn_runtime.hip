@@ -321,10 +321,10 @@ template <class H, class F, class P>
 static int build_tables_753(PlanBase &plan, DeviceBuffer &full_tab, const H &w, unsigned log_n, size_t n)
 {
     (void)plan;
-    // full table w_n^e (e < n) = hi[e >> h] * lo[e & (2^h - 1)], built on the device
+    // full table of twiddle records of w_n^e (e < n), built on the device
+    // from hi[e >> h] (canonical residues) and lo[e & (2^h - 1)] (records)
     const unsigned h = (log_n + 1) / 2;
     const size_t nlo = (size_t)1 << h, nhi = n >> h;
-    const H rk = H::from_raw(P::RK_RAW);
     std::vector<TW753> lo(nlo);
     std::vector<E753> hi(nhi);
     H acc = H::one();
@@ -335,8 +335,7 @@ static int build_tables_753(PlanBase &plan, DeviceBuffer &full_tab, const H &w, 
     const H step = acc;
     acc = H::one();
     for (size_t y = 0; y < nhi; ++y) {
-        const H v = acc * rk;  // w^(y 2^h) * 2^754, canonical
-        std::memcpy(hi[y].v, v.im_rep, sizeof(hi[y].v));
+        acc.to_canonical(hi[y].v);  // w^(y 2^h)
         acc *= step;
     }
     DeviceBuffer dlo, dhi;

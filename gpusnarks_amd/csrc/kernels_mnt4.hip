@@ -19,8 +19,11 @@ static void launch753(const PassJob<FMnt4> &single, const PassJob<FMnt4> *jobs, 
     case 8:
         hipLaunchKernelGGL((ntt753_kernel<FMnt4, 8>), dim3(grid), dim3(128), 0, st, single, jobs, tile_job);
         break;
-    default:
+    case 9:
         hipLaunchKernelGGL((ntt753_kernel<FMnt4, 9>), dim3(grid), dim3(256), 0, st, single, jobs, tile_job);
+        break;
+    default:
+        hipLaunchKernelGGL((ntt753_kernel<FMnt4, 10>), dim3(grid), dim3(512), 0, st, single, jobs, tile_job);
         break;
     }
 }

@@ -23,17 +23,18 @@
 // Two kernel families share that structure:
 //   * ntt_pass_kernel<F32, LOG_R, LOG_W, LOG_E>: compile-time tile shape,
 //     radix-16 register steps (32-bit field);
-//   * ntt753_kernel<F>: runtime tile shape (R*W <= 512, 256 threads, radix-2
-//     steps) for the 753-bit fields, whose cost is the multiplier, not the
-//     indexing.
+//   * ntt753_kernel<F, LOG_TILE>: runtime tile shape (R*W <= 2^LOG_TILE,
+//     2^(LOG_TILE-1) threads, radix-2 steps) for the 753-bit fields, whose
+//     cost is the multiplier, not the indexing.
 // Each has a multi-job form: blockIdx -> (job, tile) through a tile map, so
 // one launch can run a pass of many transforms of different sizes.
 //
 // Twiddles are device-resident tables built once per plan:
-//   column table: entry (x << col_shift) holds w_R^x;
-//   pass table:   either a full table of w_n^e (e < n), or, when
-//                 tw_pass_lo is set, a two-level split
-//                 w_n^e = hi[e >> lo_bits] * lo[e & (2^lo_bits - 1)].
+//   32-bit field, per pass: a column table (entry x holds w_R^x) and a pass
+//     table, either direct (w_{Ns R}^e) or, above 2^12 entries, a two-level
+//     split w^e = hi[e >> lo_bits] * lo[e & (2^lo_bits - 1)];
+//   753-bit fields: one full table of w_n^e (e < n); the pass twiddles are
+//     folded into the radix-2 step twiddles (see tile753_body).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -58,23 +59,6 @@ __host__ __device__ constexpr int brev_bits(int v, int bits)
     for (int i = 0; i < bits; ++i)
         r |= ((v >> i) & 1) << (bits - 1 - i);
     return r;
-}
-
-template <class F>
-__device__ __forceinline__ typename F::elem_t pass_twiddle(const typename F::elem_t &x, uint32_t e,
-                                                          const PassArgs<F> &a)
-{
-    // the two-level split is only ever set up for the 32-bit field
-    if constexpr (F::LIMBS > 1) {
-        return F::mul(x, F::load_tw(a.tw_pass, e));
-    } else {
-        // branch-free: entry 0 of every table is exactly 1, so a zero index
-        // multiplies by one instead of diverging the wave
-        if (a.tw_pass_lo == nullptr)
-            return F::mul(x, F::load_tw(a.tw_pass, e));
-        const uint32_t hi = e >> a.lo_bits, lo = e & ((1u << a.lo_bits) - 1u);
-        return F::mul(F::mul(x, F::load_tw(a.tw_pass, hi)), F::load_tw(a.tw_pass_lo, lo));
-    }
 }
 
 // ------------------------------------------------ 32-bit field, compile-time tile shape
@@ -302,52 +286,61 @@ __global__ __launch_bounds__(1 << (LOG_R + LOG_W - LOG_E)) void ntt_pass_multi_k
 
 // ------------------------------------------------ 753-bit, runtime tile shape
 
-// One tile of R = 2^log_r rows by W = 2^log_w columns (log_r + log_w <= 9)
-// with radix-2 steps.  Threads past R*W/2 only join the barriers.
+// One tile of R = 2^log_r rows by W = 2^log_w columns (log_r + log_w <=
+// LOG_TILE) with radix-2 steps.  Threads past R*W/2 only join the barriers.
+//
+// The pass twiddle w_{Ns R}^(r k) is not applied to the inputs: folded into
+// the steps, step s multiplies x1 by w_{Ns 2^(s+1)}^(k + Ns kk), kk = jj mod
+// 2^s (for Ns = 1, k = 0 these are the plain radix-2 twiddles), so a pass of
+// radix 2^r costs r products per butterfly column, not r + 1.  All come from
+// the plan's full table w_n^e, e = (k + Ns kk) n / (Ns 2^(s+1)) < n/2.
+//
+// Every butterfly is (x0 + t, x0 + 3p - t) with t = x1 * w in [0, 3p); a
+// trivial twiddle is the table's exact 1, so t is always a product (only
+// step 0 of the first pass, whose x1 is a canonical input below p, skips
+// it).  A stage adds at most 3p to the value bound.  The next step's
+// twiddle is requested as soon as the current product is done, so its
+// latency hides behind the sum, the LDS exchange and the barrier while its
+// registers are those the product no longer needs.  (An empty asm statement
+// makes the request's index depend on the product's last limb, which keeps
+// the compiler from hoisting the load into the product.)
 template <class F, int LOG_TILE>
 __device__ __forceinline__ void tile753_body(const PassArgs<F> &a, int log_r, int log_w, size_t blk)
 {
     using elem = typename F::elem_t;
+    using tw = typename F::tw_t;
     constexpr int STRIDE = lds_padded_words(1 << LOG_TILE);  // quads per plane
-    constexpr int TW_STRIDE = (1 << LOG_TILE) / 2;
     __shared__ __align__(16) uint32_t lds[STRIDE * F::LDS_WORDS_PER_ELEM];
-    // the column's twiddles w_R^x, x < R/2: seven planes of digit quads
-    // (digits 26 and 27 are padding)
-    __shared__ uint4 tw_lds[7 * TW_STRIDE];
 
     const int t = threadIdx.x;
     const int active = 1 << (log_r + log_w - 1);
     const bool on = t < active;
     const int R = 1 << log_r, W = 1 << log_w, half = R >> 1;
-    const uint32_t L = a.log_n;
+    const uint32_t L = a.log_n, log_ns = a.log_ns;
     const size_t nR = (size_t)1 << (L - log_r);
-    const size_t Ns = (size_t)1 << a.log_ns;
+    const size_t Ns = (size_t)1 << log_ns;
     const size_t j0 = blk * (size_t)W;
     const int c = t & (W - 1);
     const int jj = t >> log_w;  // in [0, R/2) for active threads
     const size_t j = j0 + c;
     const uint32_t k = (uint32_t)(j & (Ns - 1));
-
-    // stage the column twiddles; the first barrier of the step loop orders
-    // these stores before any read
-    if (t < half && log_r > 1) {
-        const TW753 w = F::load_tw(a.tw_col, (size_t)t << a.col_shift);
-#pragma unroll
-        for (int q = 0; q < 7; ++q)
-            tw_lds[q * TW_STRIDE + t] = make_uint4(w.d[4 * q], w.d[4 * q + 1], q < 6 ? w.d[4 * q + 2] : 0u,
-                                                   q < 6 ? w.d[4 * q + 3] : 0u);
-    }
+    // (step log_r, past the last, reads entry 0: a harmless, branch-free load)
+    auto step_tw = [&](int s, uint32_t after) {
+        const uint32_t kk = (uint32_t)jj & ((1u << s) - 1u);
+        uint32_t idx = s < log_r ? (k + (kk << log_ns)) << (L - log_ns - (uint32_t)s - 1u) : 0u;
+        asm volatile("" : "+v"(idx) : "v"(after));
+        return F::load_tw(a.tw_col, (size_t)idx);
+    };
 
     elem x[2];
+    tw w;  // twiddle of the current step
     if (on) {
-        // step 0: rows jj and jj + R/2 from global memory, pass twiddles first
+        // step 0: rows jj and jj + R/2 from global memory
         x[1] = F::load(a.in, j + (size_t)(jj + half) * nR);
-        const uint32_t sh = a.pass_shift;
-        if (k != 0)
-            x[1] = pass_twiddle<F>(x[1], ((uint32_t)(jj + half) * k) << sh, a);
         x[0] = F::load(a.in, j + (size_t)jj * nR);
-        if (k != 0 && jj != 0)
-            x[0] = pass_twiddle<F>(x[0], ((uint32_t)jj * k) << sh, a);
+        if (log_ns != 0)
+            x[1] = F::mul(x[1], F::load_tw(a.tw_col, (size_t)(k << (L - log_ns - 1u))));
+        w = step_tw(1, x[1].v[23]);
         const elem s0 = F::add(x[0], x[1]);
         x[1] = F::sub(x[0], x[1]);
         x[0] = s0;
@@ -363,23 +356,8 @@ __device__ __forceinline__ void tile753_body(const PassArgs<F> &a, int log_r, in
         }
         __syncthreads();
         if (on) {
-            const int kk = jj & ((1 << s) - 1);
-            x[1] = F::lds_load(lds, STRIDE, (jj + half) * W + c);
-            if (kk != 0) {
-                const int xi = kk << (log_r - s - 1);
-                TW753 w;
-#pragma unroll
-                for (int q = 0; q < 7; ++q) {
-                    const uint4 v = tw_lds[q * TW_STRIDE + xi];
-                    w.d[4 * q] = v.x;
-                    w.d[4 * q + 1] = v.y;
-                    if (q < 6) {
-                        w.d[4 * q + 2] = v.z;
-                        w.d[4 * q + 3] = v.w;
-                    }
-                }
-                x[1] = F::mul(x[1], w);
-            }
+            x[1] = F::mul(F::lds_load(lds, STRIDE, (jj + half) * W + c), w);
+            w = step_tw(s + 1, x[1].v[23]);
             x[0] = F::lds_load(lds, STRIDE, jj * W + c);
             const elem s0 = F::add(x[0], x[1]);
             x[1] = F::sub(x[0], x[1]);
@@ -454,9 +432,10 @@ __device__ __forceinline__ void tile753_body(const PassArgs<F> &a, int log_r, in
 // Single job (jobs == nullptr) or many: blockIdx -> tile_job[] -> job.
 // LOG_TILE sets the workgroup: 2^(LOG_TILE-1) threads for tiles of up to
 // 2^LOG_TILE elements.  A one-wave workgroup (LOG_TILE = 7) never waits on
-// another wave at its barriers.
+// another wave at its barriers.  Two waves per SIMD is the register budget
+// (256 VGPRs): a single transform of 2^18 has no more threads than that.
 template <class F, int LOG_TILE>
-__global__ __launch_bounds__(1 << (LOG_TILE - 1)) void ntt753_kernel(PassJob<F> single,
+__global__ __launch_bounds__(1 << (LOG_TILE - 1)) __attribute__((amdgpu_waves_per_eu(2))) void ntt753_kernel(PassJob<F> single,
                                                                    const PassJob<F> *__restrict__ jobs,
                                                                    const uint32_t *__restrict__ tile_job)
 {
@@ -469,9 +448,9 @@ __global__ __launch_bounds__(1 << (LOG_TILE - 1)) void ntt753_kernel(PassJob<F> 
     }
 }
 
-// Full 753-bit twiddle table: out[e] = hi[e >> h] * lo[e & (2^h - 1)],
-// canonical, as 29-bit digits.  hi values are < 2p in 24 limbs, lo are
-// canonical twiddles; both carry the 2^754 Montgomery factor.
+// Full 753-bit twiddle table: entry e is the twiddle record of
+// w_n^e = hi[e >> h] * lo[e & (2^h - 1)] (hi canonical residues, lo twiddle
+// records), reduced to [0, p).
 template <class F>
 __global__ __launch_bounds__(256) void build_tw753_kernel(TW753 *out, const E753 *hi, const TW753 *lo, uint32_t h,
                                                          size_t n)
@@ -479,10 +458,7 @@ __global__ __launch_bounds__(256) void build_tw753_kernel(TW753 *out, const E753
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n)
         return;
-    const E753 v = F::canon(F::mul(hi[e >> h], lo[e & ((1u << h) - 1u)]));
-    TW753 r;
-    F::to29(r.d, v.v);
-    out[e] = r;
+    out[e] = F::make_tw(F::canon(F::mul(hi[e >> h], lo[e & ((1u << h) - 1u)])));
 }
 
 // Elementwise: out[i] = canon(in[i] * s) (used for n <= 1 and scaling).

@@ -72,19 +72,19 @@ bool valid_root(const H &w, unsigned log_n)
     return x == -H::one();  // then x^2 = 1 as well
 }
 
-// w * 2^754 mod p, canonical, as 29-bit digits (the device multiplier's domain)
+// The device twiddle record of the field element w (given in Montgomery
+// form): its residue as 29-bit digits and wq = floor(w 2^783 / p).
 template <class H, class P>
 TW753 make_tw753(const H &w)
 {
-    const H rk = H::from_raw(P::RK_RAW);
-    const H v = w * rk;
+    uint32_t v[24], rem[24], d[27];
+    w.to_canonical(v);
+    (w * H::from_canonical(P::C783)).to_canonical(rem);  // w 2^783 mod p
     TW753 t{};
-    for (int i = 0; i < 26; ++i) {
-        const int bit = 29 * i, wi = bit >> 5, sh = bit & 31;
-        const uint64_t lo = v.im_rep[wi];
-        const uint64_t hi = (wi + 1 < 24) ? v.im_rep[wi + 1] : 0;
-        t.d[i] = (uint32_t)(((hi << 32) | lo) >> sh) & ((1u << 29) - 1u);
-    }
+    limbs_to_digits27(d, v);
+    for (int i = 0; i < 26; ++i)
+        t.w[i] = d[i];
+    shoup_wq<P>(t.wq, rem);
     return t;
 }
 
@@ -171,12 +171,14 @@ struct PlanImpl final : PlanBase {
                 a.lo_bits = pass_lo_bits[i];
                 a.pass_shift = 0;
             } else {
+                // the 753-bit kernel folds the pass twiddles into its step
+                // twiddles, all read from the full table
                 a.tw_col = (const tw *)full_tab.ptr;
-                a.col_shift = log_n - (uint32_t)c.log_r;
-                a.tw_pass = (const tw *)full_tab.ptr;
+                a.col_shift = 0;
+                a.tw_pass = nullptr;
                 a.tw_pass_lo = nullptr;
                 a.lo_bits = 0;
-                a.pass_shift = log_n - log_ns - (uint32_t)c.log_r;
+                a.pass_shift = 0;
             }
             a.log_n = log_n;
             a.log_ns = log_ns;
