@@ -81,3 +81,33 @@ def test_cpp_drop_in(tmp_path):
     r = subprocess.run([exe, "host"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "host: 0 failure(s)" in r.stdout
+
+
+def _header_array(struct, name):
+    text = open(os.path.join(ROOT, "include", "gsn_field_params.h")).read()
+    body = text[text.index("struct %s {" % struct):]
+    body = body[:body.index("\n};")]
+    m = re.search(r"\b%s\[\d+\] = \{(.*?)\}" % name, body, re.S)
+    return [int(v.strip().rstrip("u"), 16) for v in m.group(1).split(",")]
+
+
+@pytest.mark.parametrize("struct", ["Mnt4753FrParams", "Mnt6753FrParams"])
+def test_quotient_estimation_constants(struct):
+    """The constants of the device's 753-bit product (field_core.h): 3p,
+    C783 = 2^783 mod p (limbs and 29-bit digits), its wq = floor(C783 2^783 /
+    p), and p^-1 mod 2^783; checked with Python integers from the modulus."""
+    limbs = lambda v, bits: sum(x << (bits * i) for i, x in enumerate(v))
+    p = limbs(_header_array(struct, "MOD"), 32)
+    assert limbs(_header_array(struct, "MOD29"), 29) == p
+    assert limbs(_header_array(struct, "THREE_P"), 32) == 3 * p
+    c = limbs(_header_array(struct, "C783"), 32)
+    assert c == (1 << 783) % p and limbs(_header_array(struct, "C783_29"), 29) == c
+    assert limbs(_header_array(struct, "C783_WQ"), 29) == (c << 783) // p
+    pinv = limbs(_header_array(struct, "PINV783"), 29)
+    assert (pinv * p) % (1 << 783) == 1
+    # the exact-division identity the table builder uses for every twiddle
+    for w in (1, 2, p - 1, 12345678901234567890 % p, (p * 7) // 11):
+        rem = (w << 783) % p
+        assert ((((1 << 783) - rem) * pinv) % (1 << 783)) == (w << 783) // p
+    # the value bound the kernels rely on: 91 p < 2^761 (30 stages of +3p)
+    assert 91 * p < (1 << 761) and 3 * p < (1 << 755)
