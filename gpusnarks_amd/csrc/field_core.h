@@ -355,6 +355,30 @@ struct F753 {
             r.pad[k] = 0;
         return r;
     }
+    // record i of a table of 14 planes of `stride` quads each
+    __device__ __forceinline__ static TW753 load_tw_planes(const uint4 *tab, uint32_t stride, uint32_t i)
+    {
+        uint32_t d[56];
+#pragma unroll
+        for (int q = 0; q < 14; ++q) {
+            const uint4 v = tab[(size_t)q * stride + i];
+            d[4 * q] = v.x;
+            d[4 * q + 1] = v.y;
+            d[4 * q + 2] = v.z;
+            d[4 * q + 3] = v.w;
+        }
+        TW753 r;
+#pragma unroll
+        for (int k = 0; k < 26; ++k)
+            r.w[k] = d[k];
+#pragma unroll
+        for (int k = 0; k < 27; ++k)
+            r.wq[k] = d[26 + k];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            r.pad[k] = 0;
+        return r;
+    }
     // LDS: six planes of 16-byte quads; limbs 4q..4q+3 of element i sit in
     // quad q * stride + lds_pad(i), so an element moves with six
     // ds_write_b128 / ds_read_b128 and lanes touching consecutive elements

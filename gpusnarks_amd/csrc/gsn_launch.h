@@ -27,6 +27,7 @@ struct PassArgs {
     uint32_t col_shift;   // column table index = exponent << col_shift
     uint32_t pass_shift;  // pass table exponent = (r * k) << pass_shift
     uint32_t lo_bits;     // two-level pass table split (0: one table)
+    uint32_t tw_stride;   // 753-bit: quads per plane of the pass's step table
     uint32_t flags;
     typename F::tw_t scale;
 };
@@ -69,7 +70,10 @@ template <class F>
 void launch_scale(const typename F::elem_t *in, typename F::elem_t *out, typename F::tw_t s, size_t n,
                   int do_mul, hipStream_t stream);
 
+// Fills the step table of one 753-bit pass (see build_steps753_kernel):
+// nrec = Ns (2^log_r - 1) twiddle records in 14 planes of nrec quads.
 template <class F>
-void launch_build_tw753(TW753 *out, const E753 *hi, const TW753 *lo, uint32_t h, size_t n, hipStream_t stream);
+void launch_build_steps753(uint4 *out, uint32_t nrec, const E753 *hi, const TW753 *lo, uint32_t h, uint32_t log_n,
+                           uint32_t log_ns, uint32_t log_r, hipStream_t stream);
 
 }  // namespace gsn

@@ -10,6 +10,7 @@
 
 #include <dlfcn.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cstdio>
 #include <cstdlib>
@@ -317,14 +318,13 @@ int PlanImpl<GSN_FIELD_FP32>::build_tables(const fields::Fp32 &w, size_t n)
     return GSN_OK;
 }
 
-template <class H, class F, class P>
-static int build_tables_753(PlanBase &plan, DeviceBuffer &full_tab, const H &w, unsigned log_n, size_t n)
+template <class Plan, class H, class F, class P>
+static int build_tables_753(Plan &plan, const H &w, unsigned log_n, size_t n)
 {
-    (void)plan;
-    // full table of twiddle records of w_n^e (e < n), built on the device
-    // from hi[e >> h] (canonical residues) and lo[e & (2^h - 1)] (records)
+    // every record is w_n^e = hi[e >> h] * lo[e & (2^h - 1)] for e < n/2,
+    // computed on the device from hi (canonical residues) and lo (records)
     const unsigned h = (log_n + 1) / 2;
-    const size_t nlo = (size_t)1 << h, nhi = n >> h;
+    const size_t nlo = (size_t)1 << h, nhi = std::max<size_t>(1, (n / 2) >> h);
     std::vector<TW753> lo(nlo);
     std::vector<E753> hi(nhi);
     H acc = H::one();
@@ -343,21 +343,32 @@ static int build_tables_753(PlanBase &plan, DeviceBuffer &full_tab, const H &w, 
     GSN_TRY(dhi.alloc(nhi * sizeof(E753)));
     GSN_TRY(hip_status(hipMemcpy(dlo.ptr, lo.data(), nlo * sizeof(TW753), hipMemcpyHostToDevice), "hipMemcpy"));
     GSN_TRY(hip_status(hipMemcpy(dhi.ptr, hi.data(), nhi * sizeof(E753), hipMemcpyHostToDevice), "hipMemcpy"));
-    GSN_TRY(full_tab.alloc(n * sizeof(TW753)));
-    launch_build_tw753<F>((TW753 *)full_tab.ptr, (const E753 *)dhi.ptr, (const TW753 *)dlo.ptr, h, n, 0);
-    GSN_TRY(hip_status(hipGetLastError(), "build_tw753 launch"));
-    return hip_status(hipStreamSynchronize(nullptr), "build_tw753");
+    const size_t P_ = plan.cfg.size();
+    plan.step_tabs.resize(P_);
+    plan.step_nrec.assign(P_, 0);
+    unsigned log_ns = 0;
+    for (size_t i = 0; i < P_; ++i) {
+        const unsigned lr = (unsigned)plan.cfg[i].log_r;
+        const uint32_t nrec = (uint32_t)((((size_t)1 << lr) - 1) << log_ns);
+        GSN_TRY(plan.step_tabs[i].alloc((size_t)nrec * sizeof(TW753)));
+        plan.step_nrec[i] = nrec;
+        launch_build_steps753<F>((uint4 *)plan.step_tabs[i].ptr, nrec, (const E753 *)dhi.ptr, (const TW753 *)dlo.ptr,
+                                 h, log_n, log_ns, lr, 0);
+        GSN_TRY(hip_status(hipGetLastError(), "build_steps753 launch"));
+        log_ns += lr;
+    }
+    return hip_status(hipStreamSynchronize(nullptr), "build_steps753");
 }
 
 template <>
 int PlanImpl<GSN_FIELD_MNT4753_FR>::build_tables(const fields::Scalar &w, size_t n)
 {
-    return build_tables_753<fields::Scalar, FMnt4, Mnt4753FrParams>(*this, full_tab, w, log_n, n);
+    return build_tables_753<PlanImpl, fields::Scalar, FMnt4, Mnt4753FrParams>(*this, w, log_n, n);
 }
 template <>
 int PlanImpl<GSN_FIELD_MNT6753_FR>::build_tables(const fields::ScalarMNT6 &w, size_t n)
 {
-    return build_tables_753<fields::ScalarMNT6, FMnt6, Mnt6753FrParams>(*this, full_tab, w, log_n, n);
+    return build_tables_753<PlanImpl, fields::ScalarMNT6, FMnt6, Mnt6753FrParams>(*this, w, log_n, n);
 }
 
 template <int FID>

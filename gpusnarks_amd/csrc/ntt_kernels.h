@@ -33,8 +33,9 @@
 //   32-bit field, per pass: a column table (entry x holds w_R^x) and a pass
 //     table, either direct (w_{Ns R}^e) or, above 2^12 entries, a two-level
 //     split w^e = hi[e >> lo_bits] * lo[e & (2^lo_bits - 1)];
-//   753-bit fields: one full table of w_n^e (e < n); the pass twiddles are
-//     folded into the radix-2 step twiddles (see tile753_body).
+//   753-bit fields, per pass: a step table of the radix-2 step twiddles with
+//     the pass twiddles folded in (see tile753_body), n - 1 records over all
+//     passes.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -292,8 +293,7 @@ __global__ __launch_bounds__(1 << (LOG_R + LOG_W - LOG_E)) void ntt_pass_multi_k
 // The pass twiddle w_{Ns R}^(r k) is not applied to the inputs: folded into
 // the steps, step s multiplies x1 by w_{Ns 2^(s+1)}^(k + Ns kk), kk = jj mod
 // 2^s (for Ns = 1, k = 0 these are the plain radix-2 twiddles), so a pass of
-// radix 2^r costs r products per butterfly column, not r + 1.  All come from
-// the plan's full table w_n^e, e = (k + Ns kk) n / (Ns 2^(s+1)) < n/2.
+// radix 2^r costs r products per butterfly column, not r + 1.
 //
 // Every butterfly is (x0 + t, x0 + 3p - t) with t = x1 * w in [0, 3p); a
 // trivial twiddle is the table's exact 1, so t is always a product (only
@@ -303,7 +303,8 @@ __global__ __launch_bounds__(1 << (LOG_R + LOG_W - LOG_E)) void ntt_pass_multi_k
 // latency hides behind the sum, the LDS exchange and the barrier while its
 // registers are those the product no longer needs.  (An empty asm statement
 // makes the request's index depend on the product's last limb, which keeps
-// the compiler from hoisting the load into the product.)
+// the compiler from hoisting the load into the product.)  The twiddles come
+// from a per-pass step table laid out in the order the lanes read them.
 template <class F, int LOG_TILE>
 __device__ __forceinline__ void tile753_body(const PassArgs<F> &a, int log_r, int log_w, size_t blk)
 {
@@ -324,12 +325,16 @@ __device__ __forceinline__ void tile753_body(const PassArgs<F> &a, int log_r, in
     const int jj = t >> log_w;  // in [0, R/2) for active threads
     const size_t j = j0 + c;
     const uint32_t k = (uint32_t)(j & (Ns - 1));
-    // (step log_r, past the last, reads entry 0: a harmless, branch-free load)
+    // the pass's step table (build_steps753_kernel): step s, column k, kk
+    // at (2^s - 1) Ns + k 2^s + kk; step log_r, past the last, reads record
+    // 0 (a harmless, branch-free load)
+    const uint4 *tab = reinterpret_cast<const uint4 *>(a.tw_col);
+    const uint32_t planes = a.tw_stride;
     auto step_tw = [&](int s, uint32_t after) {
         const uint32_t kk = (uint32_t)jj & ((1u << s) - 1u);
-        uint32_t idx = s < log_r ? (k + (kk << log_ns)) << (L - log_ns - (uint32_t)s - 1u) : 0u;
+        uint32_t idx = s < log_r ? (((1u << s) - 1u) << log_ns) + (k << s) + kk : 0u;
         asm volatile("" : "+v"(idx) : "v"(after));
-        return F::load_tw(a.tw_col, (size_t)idx);
+        return F::load_tw_planes(tab, planes, idx);
     };
 
     elem x[2];
@@ -339,7 +344,7 @@ __device__ __forceinline__ void tile753_body(const PassArgs<F> &a, int log_r, in
         x[1] = F::load(a.in, j + (size_t)(jj + half) * nR);
         x[0] = F::load(a.in, j + (size_t)jj * nR);
         if (log_ns != 0)
-            x[1] = F::mul(x[1], F::load_tw(a.tw_col, (size_t)(k << (L - log_ns - 1u))));
+            x[1] = F::mul(x[1], F::load_tw_planes(tab, planes, k));
         w = step_tw(1, x[1].v[23]);
         const elem s0 = F::add(x[0], x[1]);
         x[1] = F::sub(x[0], x[1]);
@@ -448,17 +453,31 @@ __global__ __launch_bounds__(1 << (LOG_TILE - 1)) __attribute__((amdgpu_waves_pe
     }
 }
 
-// Full 753-bit twiddle table: entry e is the twiddle record of
-// w_n^e = hi[e >> h] * lo[e & (2^h - 1)] (hi canonical residues, lo twiddle
-// records), reduced to [0, p).
+// Step table of one 753-bit pass (log_ns, log_r) of a 2^log_n transform:
+// the records tile753_body reads, in the order a wave reads them, stored as
+// 14 planes of 16-byte quads (quad q of record i at q * nrec + i) so that a
+// wave's request for one quad of 64 consecutive records is one contiguous
+// kilobyte.  Step s (2^s Ns records from offset (2^s - 1) Ns) holds, at
+// k 2^s + kk, w_{Ns 2^(s+1)}^(k + Ns kk) = w_n^e with
+// e = (k + Ns kk) << (log_n - log_ns - s - 1) < n/2; each record is
+// w_n^e = hi[e >> h] * lo[e & (2^h - 1)] reduced to [0, p).
 template <class F>
-__global__ __launch_bounds__(256) void build_tw753_kernel(TW753 *out, const E753 *hi, const TW753 *lo, uint32_t h,
-                                                         size_t n)
+__global__ __launch_bounds__(256) void build_steps753_kernel(uint4 *out, uint32_t nrec, const E753 *hi, const TW753 *lo,
+                                                            uint32_t h, uint32_t log_n, uint32_t log_ns, uint32_t log_r)
 {
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n)
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrec)
         return;
-    out[e] = F::make_tw(F::canon(F::mul(hi[e >> h], lo[e & ((1u << h) - 1u)])));
+    const uint32_t s = 31u - (uint32_t)__builtin_clz((i >> log_ns) + 1u);
+    const uint32_t jx = i - (((1u << s) - 1u) << log_ns);
+    const uint32_t k = jx >> s, kk = jx & ((1u << s) - 1u);
+    const uint32_t e = (k + (kk << log_ns)) << (log_n - log_ns - s - 1u);
+    const TW753 r = F::make_tw(F::canon(F::mul(hi[e >> h], lo[e & ((1u << h) - 1u)])));
+    const uint4 *q = reinterpret_cast<const uint4 *>(&r);
+#pragma unroll
+    for (int p = 0; p < 14; ++p)
+        out[(size_t)p * nrec + i] = q[p];
+    (void)log_r;
 }
 
 // Elementwise: out[i] = canon(in[i] * s) (used for n <= 1 and scaling).

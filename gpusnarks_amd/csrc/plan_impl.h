@@ -100,7 +100,9 @@ struct PlanImpl final : PlanBase {
     static constexpr bool IS32 = (FID == GSN_FIELD_FP32);
 
     std::vector<PassCfg> cfg;
-    DeviceBuffer full_tab;               // 753-bit: w_n^e for e < n (all twiddles)
+    // 753-bit, per pass: step table of Ns (2^r - 1) records in 14 planes
+    std::vector<DeviceBuffer> step_tabs;
+    std::vector<uint32_t> step_nrec;
     // 32-bit, per pass: column table w_R^x (x < R) and pass table in units
     // of w_{Ns R} -- direct, or hi/lo halves of a two-level split
     std::vector<DeviceBuffer> col_tabs, pass_tabs, pass_los;
@@ -123,8 +125,8 @@ struct PlanImpl final : PlanBase {
         if (log_n == 0)
             return GSN_OK;
         GSN_TRY(build_tables(w, n));
-        device_bytes = full_tab.bytes;
-        for (auto *v : {&col_tabs, &pass_tabs, &pass_los})
+        device_bytes = 0;
+        for (auto *v : {&col_tabs, &pass_tabs, &pass_los, &step_tabs})
             for (auto &b : *v)
                 device_bytes += b.bytes;
         return GSN_OK;
@@ -172,8 +174,9 @@ struct PlanImpl final : PlanBase {
                 a.pass_shift = 0;
             } else {
                 // the 753-bit kernel folds the pass twiddles into its step
-                // twiddles, all read from the full table
-                a.tw_col = (const tw *)full_tab.ptr;
+                // twiddles, read from the pass's step table
+                a.tw_col = (const tw *)step_tabs[i].ptr;
+                a.tw_stride = step_nrec[i];
                 a.col_shift = 0;
                 a.tw_pass = nullptr;
                 a.tw_pass_lo = nullptr;
