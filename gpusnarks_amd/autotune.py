@@ -27,6 +27,7 @@ DEFAULT_SIZES = {"fp32": range(10, 29), "mnt4": range(10, 16), "mnt6": range(10,
 # compiled 32-bit tile shapes (gpusnarks_amd/csrc/kernels_fp32.hip)
 F32_SINGLE = {r: (r, 0, min(4, r)) for r in range(1, 13)}
 F32_MULTI = {r: (r, 4, 4) for r in range(5, 11)}
+F32_WIDE = {r: (r, 5, 4) for r in range(5, 10)}
 
 
 def compositions(total, parts, lo, hi):
@@ -49,6 +50,8 @@ def candidates(field, log_n):
             for comp in compositions(log_n, P, 5, 10):
                 if all(log_n - r >= 4 for r in comp):
                     out.append([F32_MULTI[r] for r in comp])
+                if all(log_n - r >= 5 and r <= 9 for r in comp):
+                    out.append([F32_WIDE[r] for r in comp])
         return out
     for P in (1, 2, 3, 4):
         for comp in compositions(log_n, P, 3 if P > 1 else 1, 9):
