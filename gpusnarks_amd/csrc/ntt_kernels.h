@@ -90,6 +90,12 @@ __device__ __forceinline__ F32::tw_t ldtw32(const F32::tw_t *tab, uint32_t i)
     const uint2 v = *reinterpret_cast<const uint2 *>(reinterpret_cast<const char *>(tab) + (i << 3));
     return F32::tw_t{v.x, v.y};
 }
+// the twiddle record of the product of two twiddles (Montgomery forms)
+__device__ __forceinline__ F32::tw_t tw32_product(const F32::tw_t &u, const F32::tw_t &v)
+{
+    const uint32_t w = F32::mul(u.w, v);
+    return F32::tw_t{w, w * Fp32Params::PINV};
+}
 
 // Natural-order radix-2 DFT of ES = 2^LOG_ES register elements x[OFF..OFF+ES),
 // root w_ES = w_R^(R/ES); w_{2m}^jx = w_R^(jx * R / 2m) from the column table.
@@ -181,28 +187,33 @@ __device__ __forceinline__ void pass32_body(const PassArgs<F32> &a, uint32_t blk
         for (int e = 0; e < E0; ++e)
             x[g * E0 + e] = ld32(a.in, j + (uint32_t)(jj + g * (R / E) + e * (R / E0)) * nR);
     if (a.log_ns != 0) {
-        // branch-free within the wave: an exponent of 0 reads the table's
-        // exact 1; the table form (direct or two-level) is uniform
-        const uint32_t sh = a.pass_shift;
-        if (a.tw_pass_lo == nullptr) {
+        // The thread's rows are r = jj + g*(R/E) + e*(R/E0) of one column k,
+        // so its twiddles w^(r k) = A B^g C^e with A = w^(jj k),
+        // B = w^((R/E) k), C = w^((R/E0) k) (w = w_{Ns R}): three table
+        // lookups and E - 1 twiddle products instead of E scattered lookups
+        // (each a two-level pair above 2^12 entries), which would keep the
+        // texture path busy with one cache line per lane.  An exponent of 0
+        // reads the table's exact 1; the table form is uniform.
+        const uint32_t sh = a.pass_shift, lb = a.lo_bits, lm = (1u << lb) - 1u;
+        auto twx = [&](uint32_t ex) {
+            ex <<= sh;
+            if (a.tw_pass_lo == nullptr)
+                return ldtw32(a.tw_pass, ex);
+            return tw32_product(ldtw32(a.tw_pass, ex >> lb), ldtw32(a.tw_pass_lo, ex & lm));
+        };
+        const F::tw_t tB = twx((uint32_t)(R / E) * k), tC = twx((uint32_t)(R / E0) * k);
+        F::tw_t tg = twx((uint32_t)jj * k);
 #pragma unroll
-            for (int g = 0; g < G0; ++g)
+        for (int g = 0; g < G0; ++g) {
+            F::tw_t te = tg;
 #pragma unroll
-                for (int e = 0; e < E0; ++e) {
-                    const uint32_t r = (uint32_t)(jj + g * (R / E) + e * (R / E0));
-                    x[g * E0 + e] = F::mul(x[g * E0 + e], ldtw32(a.tw_pass, (r * k) << sh));
-                }
-        } else {
-            const uint32_t lb = a.lo_bits, lm = (1u << lb) - 1u;
-#pragma unroll
-            for (int g = 0; g < G0; ++g)
-#pragma unroll
-                for (int e = 0; e < E0; ++e) {
-                    const uint32_t r = (uint32_t)(jj + g * (R / E) + e * (R / E0));
-                    const uint32_t ex = (r * k) << sh;
-                    x[g * E0 + e] = F::mul(F::mul(x[g * E0 + e], ldtw32(a.tw_pass, ex >> lb)),
-                                           ldtw32(a.tw_pass_lo, ex & lm));
-                }
+            for (int e = 0; e < E0; ++e) {
+                x[g * E0 + e] = F::mul(x[g * E0 + e], te);
+                if (e + 1 < E0)
+                    te = tw32_product(te, tC);
+            }
+            if (g + 1 < G0)
+                tg = tw32_product(tg, tB);
         }
     }
     dft32_groups<LOG_R, LOG_E0, E, G0>(x, a.tw_col);
