@@ -349,8 +349,14 @@ def main():
         "kernel": "ntt753_kernel (one Stockham pass, radix 2^%d)" % (log_n // passes),
         "launches_per_step": passes, "avg_launch_ms": ms_launch,
         "work_per_launch": "%d butterflies x 1176 v_mad_u64_u32" % (bfly(log_n) // passes),
-        "peak_note": "256 CU x 64 lanes/clk x 2.4 GHz; a microbenchmark measured 34.7e12/s",
+        "peak_note": "256 CU x 64 lanes/clk x 2.4 GHz (peak engine clock); clock_measured_ghz is the engine "
+                     "clock while the 753-bit product runs on every CU (gsn_debug_clock_ghz)",
     }
+    ghz = ctypes.c_double(0.0)
+    g.check(g.lib().gsn_debug_clock_ghz(ctypes.byref(ghz)))
+    roofline["clock_measured_ghz"] = ghz.value
+    roofline["peak_at_measured_clock"] = MAD_PEAK / 1e12 * ghz.value * 1e9 / CLOCK_HZ
+    roofline["frac_at_measured_clock"] = achieved / (MAD_PEAK * ghz.value * 1e9 / CLOCK_HZ)
     traffic_file = os.path.join(ROOT, "profiles", "ntt753_n18_traffic.json")
     if os.path.exists(traffic_file):
         with open(traffic_file) as fh:

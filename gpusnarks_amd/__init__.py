@@ -100,6 +100,7 @@ def lib():
     L.gsn_event_synchronize.argtypes = [vp]
     L.gsn_event_elapsed_ms.argtypes = [ctypes.POINTER(ctypes.c_float), vp, vp]
     L.gsn_debug_copy_u32.argtypes = [vp, vp, sz, vp]
+    L.gsn_debug_clock_ghz.argtypes = [ctypes.POINTER(ctypes.c_double)]
     L.gsn_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(BatchItem), sz, ctypes.c_int]
     L.gsn_batch_launch.argtypes = [vp, vp]
     L.gsn_batch_destroy.argtypes = [vp]

@@ -214,7 +214,11 @@ struct F753 {
             acc >>= 29;
         }
         q[26] = (uint32_t)acc;
-        // a*w - q*p modulo 2^783: columns 0..26
+        // a*w - q*p modulo 2^783: columns 0..26.  The digit -p_0 goes
+        // through an SGPR the compiler cannot see into: for p_0 = 1 it would
+        // otherwise turn q_c * -1 into a two-instruction 64-bit subtraction.
+        int32_t neg_p0;
+        asm("s_mov_b32 %0, %1" : "=s"(neg_p0) : "i"(-(int32_t)P::MOD29[0]));
         uint32_t r[27];
         int64_t s = 0;
 #pragma unroll
@@ -224,7 +228,7 @@ struct F753 {
                 s += (int64_t)((uint64_t)a[i] * t.w[c - i]);
 #pragma unroll
             for (int i = (c > 25 ? c - 25 : 0); i <= (c < 26 ? c : 26); ++i)
-                s += (int64_t)(int32_t)q[i] * (int64_t)(-(int32_t)P::MOD29[c - i]);
+                s += (int64_t)(int32_t)q[i] * (int64_t)(c - i == 0 ? neg_p0 : -(int32_t)P::MOD29[c - i]);
             r[c] = (uint32_t)s & M29;
             s >>= 29;
         }

@@ -66,6 +66,9 @@ template <class F>
 void launch_pass_multi(const PassJob<F> *d_jobs, const uint32_t *d_tile_job, uint32_t total_tiles, const PassCfg &cfg,
                        hipStream_t stream);
 
+// gsn_debug_clock_ghz's kernel (kernels_mnt6.hip)
+void launch_clock_probe(unsigned long long *spans, E753 *sink, unsigned blocks, int iters, hipStream_t stream);
+
 template <class F>
 void launch_scale(const typename F::elem_t *in, typename F::elem_t *out, typename F::tw_t s, size_t n,
                   int do_mul, hipStream_t stream);

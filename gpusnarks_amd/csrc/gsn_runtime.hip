@@ -784,6 +784,37 @@ int gsn_debug_copy_u32(void *d_out, const void *d_in, size_t count, void *stream
     return hip_status(hipGetLastError(), "copy launch");
 }
 
+int gsn_debug_clock_ghz(double *ghz)
+{
+    if (!ghz)
+        return set_error(GSN_ERR_INVALID_ARGUMENT, "ghz is NULL");
+    GSN_TRY(ensure_device());
+    int dev = 0, cus = 0, wall_khz = 0;
+    GSN_TRY(hip_status(hipGetDevice(&dev), "hipGetDevice"));
+    GSN_TRY(hip_status(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "CU count"));
+    GSN_TRY(hip_status(hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, dev), "wall clock rate"));
+    const unsigned blocks = (unsigned)cus * 4;  // 4 waves per SIMD
+    DeviceBuffer spans, sink;
+    GSN_TRY(spans.alloc((size_t)blocks * 2 * sizeof(unsigned long long)));
+    GSN_TRY(sink.alloc(sizeof(E753)));
+    for (int rep = 0; rep < 2; ++rep) {  // the first run lets the clock settle
+        launch_clock_probe((unsigned long long *)spans.ptr, (E753 *)sink.ptr, blocks, 256, nullptr);
+        GSN_TRY(hip_status(hipGetLastError(), "clock probe launch"));
+    }
+    GSN_TRY(hip_status(hipStreamSynchronize(nullptr), "clock probe"));
+    std::vector<unsigned long long> h((size_t)blocks * 2);
+    GSN_TRY(hip_status(hipMemcpy(h.data(), spans.ptr, h.size() * sizeof(h[0]), hipMemcpyDeviceToHost), "hipMemcpy"));
+    double cyc = 0, wall = 0;
+    for (unsigned b = 0; b < blocks; ++b) {
+        cyc += (double)h[2 * b];
+        wall += (double)h[2 * b + 1];
+    }
+    if (wall <= 0 || wall_khz <= 0)
+        return set_error(GSN_ERR_DEVICE, "clock probe measured nothing");
+    *ghz = cyc / wall * (double)wall_khz * 1e-6;
+    return GSN_OK;
+}
+
 int gsn_event_elapsed_ms(float *ms, void *start, void *stop)
 {
     if (!ms)

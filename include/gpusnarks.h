@@ -101,6 +101,11 @@ int gsn_event_elapsed_ms(float *ms, void *start, void *stop);
 /* Copies count u32 words with the library's elementwise kernel (one read and
  * one write per word): a known-traffic reference for hardware counters. */
 int gsn_debug_copy_u32(void *d_out, const void *d_in, size_t count, void *stream);
+/* Runs the 753-bit product on every CU (four waves per SIMD, about a
+ * millisecond) and reports the engine clock it ran at, from the core cycle
+ * counter against the constant-rate wall clock: the clock behind a measured
+ * multiply rate, which under this load sits below the 2.4 GHz peak. */
+int gsn_debug_clock_ghz(double *ghz);
 
 /* --- plain transform: the best_fft replacement ---------------------------
  * cuda/fft_kernel.cu:117-147: host vector in, transform in place,
