@@ -82,14 +82,15 @@ def time_plan(field, log_n, steps, warmup, inverse=False):
     return ms / steps, plan.num_passes
 
 
-def cpu_baseline(log_n, threads):
+def cpu_baseline(log_n, threads, field=None, serial=True):
     """Oracle (test/fft_host.h restated) on the host: the reference
     harness's parallel radix-2 FFT with log_cpus = 10, bounded sample."""
     import ctypes
     from tests.oracle_lib import Oracle
     import gpusnarks_amd as g
     orc = Oracle()
-    field = g.FIELD_MNT6753_FR
+    field = g.FIELD_MNT6753_FR if field is None else field
+    fname = {g.FIELD_FP32: "Fp32", g.FIELD_MNT4753_FR: "MNT4-753 Fr", g.FIELD_MNT6753_FR: "MNT6-753 Fr"}[field]
     n = 1 << log_n
     x = inputs(field, log_n, 0x5EED0000 + log_n)
     om = g.root_of_unity(field, log_n)
@@ -98,7 +99,7 @@ def cpu_baseline(log_n, threads):
     # phase 1 (fft_host.h:76-102): 1024 partial sums, each O(n) products;
     # time a slice of the j range and scale
     j_sample = max(threads, 16)
-    tmp = np.zeros((j_sample * rows, 24), np.uint32)
+    tmp = np.zeros((j_sample * rows, g.NLIMBS[field]), np.uint32)
     f = orc.field(field)
     u32p = ctypes.POINTER(ctypes.c_uint32)
     t0 = time.perf_counter()
@@ -111,20 +112,50 @@ def cpu_baseline(log_n, threads):
     orc.serial_fft(field, tmp[:rows], w_c)
     t_p2 = (time.perf_counter() - t0) * (1 << log_cpus) / threads
     total = t_p1 + t_p2
-    # and the O(n log n) serial transform on one core, in full
-    t0 = time.perf_counter()
-    orc.serial_fft(field, x, om)
-    t_serial = time.perf_counter() - t0
-    return {
+    out = {
         "value": bfly(log_n) / total, "unit": "butterflies/s", "cores": threads, "kind": "port",
-        "sample": "oracle _basic_parallel_radix2_FFT_inner, MNT6-753 Fr n=2^%d, log_cpus=10: phase 1 timed "
+        "sample": "oracle _basic_parallel_radix2_FFT_inner, %s n=2^%d, log_cpus=10: phase 1 timed "
                   "for j in [0,%d) of 1024 on %d threads and scaled x%.0f; phase 2 timed for 1 of 1024 "
-                  "size-%d FFTs and scaled to %d threads" % (log_n, j_sample, threads, 1024 / j_sample, rows,
+                  "size-%d FFTs and scaled to %d threads" % (fname, log_n, j_sample, threads, 1024 / j_sample, rows,
                                                              threads),
         "seconds_extrapolated": total,
-        "serial_radix2_1core_butterflies_per_s": bfly(log_n) / t_serial,
-        "serial_radix2_1core_seconds": t_serial,
     }
+    if serial:
+        # and the O(n log n) serial transform on one core, in full
+        t0 = time.perf_counter()
+        orc.serial_fft(field, x, om)
+        t_serial = time.perf_counter() - t0
+        out["serial_radix2_1core_butterflies_per_s"] = bfly(log_n) / t_serial
+        out["serial_radix2_1core_seconds"] = t_serial
+    return out
+
+
+def bench_configs(threads, skip_cpu):
+    """Every size the configurations name, plain transforms with operands in
+    HBM: butterflies/s, elements/s and the roofline fraction of each."""
+    import gpusnarks_amd as g
+    rows = []
+    for name, field, log_n, inv in [("fp32", g.FIELD_FP32, 16, False), ("fp32", g.FIELD_FP32, 20, False),
+                                    ("fp32", g.FIELD_FP32, 24, False), ("fp32", g.FIELD_FP32, 26, False),
+                                    ("fp32", g.FIELD_FP32, 28, False), ("mnt4", g.FIELD_MNT4753_FR, 15, False),
+                                    ("mnt4", g.FIELD_MNT4753_FR, 15, True), ("mnt6", g.FIELD_MNT6753_FR, 16, False),
+                                    ("mnt6", g.FIELD_MNT6753_FR, 16, True), ("mnt6", g.FIELD_MNT6753_FR, 20, False)]:
+        steps = 20 if log_n <= 26 else 5
+        ms, passes = time_plan(field, log_n, steps, 2, inverse=inv)
+        r = {"field": name, "log2_n": log_n, "inverse": inv, "ms": ms, "passes": passes,
+             "butterflies_per_s": bfly(log_n) / (ms * 1e-3), "elements_per_s": (1 << log_n) / (ms * 1e-3)}
+        if field == g.FIELD_FP32:
+            alg = 8 * (1 << log_n) * ((log_n + 14) // 15)
+            r["algorithmic_GBps"] = alg / (ms * 1e-3) / 1e9
+            r["frac_of_8TBps"] = alg / (ms * 1e-3) / HBM_PEAK
+        else:
+            r["frac_of_mad_peak"] = bfly(log_n) * MADS_PER_BFLY_753 / (ms * 1e-3) / MAD_PEAK
+        rows.append(r)
+        log("config %s 2^%d%s: %.4f ms" % (name, log_n, " inverse" if inv else "", ms))
+    out = {"plain": rows}
+    if not skip_cpu:
+        out["cpu_fp32"] = [cpu_baseline(l, threads, g.FIELD_FP32, serial=(l == 20)) for l in (20, 22)]
+    return out
 
 
 def ragged_workload(seed=0x5EED0003, count=256):
@@ -390,6 +421,7 @@ def main():
         line["batched"] = bench_batched()
         log("batched 16 x 2^%d" % log_n)
         line["batched_same_size"] = bench_batched_same(log_n)
+        line["configs"] = bench_configs(min(16, len(os.sched_getaffinity(0))), args.skip_cpu)
         log("serving")
         line["serving"] = bench_serving()
     print(json.dumps(line))

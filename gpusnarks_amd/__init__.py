@@ -101,6 +101,11 @@ def lib():
     L.gsn_event_elapsed_ms.argtypes = [ctypes.POINTER(ctypes.c_float), vp, vp]
     L.gsn_debug_copy_u32.argtypes = [vp, vp, sz, vp]
     L.gsn_debug_clock_ghz.argtypes = [ctypes.POINTER(ctypes.c_double)]
+    L.gsn_pointwise_mul.argtypes = [ctypes.c_int, vp, vp, vp, sz, vp]
+    L.gsn_scale.argtypes = [ctypes.c_int, vp, vp, sz, vp, vp]
+    L.gsn_coset_shift.argtypes = [ctypes.c_int, vp, vp, sz, vp, ctypes.c_int, vp]
+    L.gsn_to_montgomery.argtypes = [ctypes.c_int, vp, vp, sz, vp]
+    L.gsn_from_montgomery.argtypes = [ctypes.c_int, vp, vp, sz, vp]
     L.gsn_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(BatchItem), sz, ctypes.c_int]
     L.gsn_batch_launch.argtypes = [vp, vp]
     L.gsn_batch_destroy.argtypes = [vp]
@@ -158,6 +163,42 @@ def best_fft(a, omega=None, field=FIELD_MNT6753_FR, inverse=False):
 
 def best_ifft(a, omega=None, field=FIELD_MNT6753_FR):
     return best_fft(a, omega, field, inverse=True)
+
+
+# ---- domain helpers on device buffers (pointers), asynchronous on `stream`
+
+def _elem_ptr(v, field):
+    v = np.ascontiguousarray(np.asarray(v, dtype=np.uint32).reshape(-1))
+    if v.shape[0] != NLIMBS[field]:
+        raise ValueError("expected one element of %d limbs" % NLIMBS[field])
+    return v, v.ctypes.data_as(ctypes.c_void_p)
+
+
+def pointwise_mul(field, d_out, d_a, d_b, n, stream=None):
+    """d_out[i] = d_a[i] * d_b[i] (Montgomery forms)."""
+    check(lib().gsn_pointwise_mul(field, ctypes.c_void_p(d_out), ctypes.c_void_p(d_a), ctypes.c_void_p(d_b), n,
+                                  stream))
+
+
+def scale(field, d_out, d_in, n, s, stream=None):
+    """d_out[i] = d_in[i] * s, s one element (Montgomery form)."""
+    v, vp_ = _elem_ptr(s, field)
+    check(lib().gsn_scale(field, ctypes.c_void_p(d_out), ctypes.c_void_p(d_in), n, vp_, stream))
+
+
+def coset_shift(field, d_out, d_in, n, g, inverse=False, stream=None):
+    """d_out[i] = d_in[i] * g^i (g^-i with inverse=True)."""
+    v, vp_ = _elem_ptr(g, field)
+    check(lib().gsn_coset_shift(field, ctypes.c_void_p(d_out), ctypes.c_void_p(d_in), n, vp_, int(bool(inverse)),
+                                stream))
+
+
+def to_montgomery(field, d_out, d_in, n, stream=None):
+    check(lib().gsn_to_montgomery(field, ctypes.c_void_p(d_out), ctypes.c_void_p(d_in), n, stream))
+
+
+def from_montgomery(field, d_out, d_in, n, stream=None):
+    check(lib().gsn_from_montgomery(field, ctypes.c_void_p(d_out), ctypes.c_void_p(d_in), n, stream))
 
 
 def root_of_unity(field, log_n):

@@ -1,0 +1,284 @@
+// Domain helpers around the transform: the elementwise steps that sit on
+// either side of an NTT in a prover (the reference has none of them; its
+// FFT, cuda/fft_kernel.cu:117-147, is the only operation on the path):
+//
+//   gsn_pointwise_mul    out[i] = a[i] * b[i]            (evaluation-form product)
+//   gsn_scale            out[i] = in[i] * s               (e.g. n^-1, a constant)
+//   gsn_coset_shift      out[i] = in[i] * g^(+-i)         (coset NTT: shift, transform)
+//   gsn_to_montgomery    out[i] = in[i] * R mod p         (plain residues -> API form)
+//   gsn_from_montgomery  out[i] = in[i] * R^-1 mod p      (API form -> plain residues)
+//
+// All work on device buffers (out may equal the input), are asynchronous on
+// the given stream and write canonical residues.  Every product is the
+// field's twiddle product (F32::mul, F753<P>::mul) by a precomputed record of
+// the constant factor; a coset's g^i is lo[i mod 2^h] * hi[i >> h] from two
+// tables of about sqrt(n) records, cached per (field, g, n, direction).
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "gpusnarks.h"
+#include "gsn_internal.h"
+#include "plan_impl.h"
+
+namespace gsn {
+
+// ------------------------------------------------------------ kernels
+
+// out[i] = in[i] * rec
+template <class F>
+__global__ __launch_bounds__(256) void ew_scale_kernel(typename F::elem_t *out, const typename F::elem_t *in,
+                                                      typename F::tw_t rec, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        F::store(out, i, F::canon(F::mul(F::load(in, i), rec)));
+}
+
+// out[i] = in[i] * lo[i & (2^h - 1)] * hi[i >> h]
+template <class F>
+__global__ __launch_bounds__(256) void ew_coset_kernel(typename F::elem_t *out, const typename F::elem_t *in,
+                                                      const typename F::tw_t *lo, const typename F::tw_t *hi,
+                                                      uint32_t h, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const typename F::elem_t v = F::mul(F::load(in, i), F::load_tw(lo, i & ((1u << h) - 1u)));
+    F::store(out, i, F::canon(F::mul(v, F::load_tw(hi, i >> h))));
+}
+
+// out[i] = a[i] * b[i]: b[i] becomes a twiddle record first
+__device__ __forceinline__ uint32_t pointwise_one(uint32_t a, uint32_t b)
+{
+    return F32::mul(a, F32::tw_t{b, b * Fp32Params::PINV});
+}
+template <class P>
+__device__ __forceinline__ E753 pointwise_one(const E753 &a, const E753 &b, const TW753 &rinv)
+{
+    // b = b' R (Montgomery form): its value b' = b * R^-1, as a record
+    const TW753 rec = F753<P>::make_tw(F753<P>::canon(F753<P>::mul(b, rinv)));
+    return F753<P>::canon(F753<P>::mul(a, rec));
+}
+
+__global__ __launch_bounds__(256) void ew_pointwise32_kernel(uint32_t *out, const uint32_t *a, const uint32_t *b,
+                                                            size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        out[i] = pointwise_one(a[i], b[i]);
+}
+
+template <class P>
+__global__ __launch_bounds__(256) void ew_pointwise753_kernel(E753 *out, const E753 *a, const E753 *b, TW753 rinv,
+                                                             size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        F753<P>::store(out, i, pointwise_one<P>(F753<P>::load(a, i), F753<P>::load(b, i), rinv));
+}
+
+// ------------------------------------------------------------ host side
+
+namespace {
+
+unsigned blocks_for(size_t n) { return (unsigned)((n + 255) / 256); }
+
+template <int FID>
+typename FieldInfo<FID>::Dev::tw_t record(const typename FieldInfo<FID>::Host &v)
+{
+    return PlanImpl<FID>::make_tw(v);
+}
+
+// value R mod p and R^-1 mod p as field elements
+template <int FID>
+typename FieldInfo<FID>::Host value_r()
+{
+    using H = typename FieldInfo<FID>::Host;
+    return H::from_canonical(H::one().im_rep);  // one() holds R mod p
+}
+
+struct CosetTables {
+    DeviceBuffer lo, hi;
+    uint32_t h = 0;
+};
+
+std::mutex g_coset_mutex;
+std::map<std::tuple<int, unsigned, int, std::vector<uint32_t>>, std::unique_ptr<CosetTables>> g_coset;
+
+// g^(+-i) = lo[i & (2^h - 1)] * hi[i >> h] for i < 2^log_n
+template <int FID>
+int coset_tables(CosetTables **out, const typename FieldInfo<FID>::Host &g, unsigned log_n, int inverse)
+{
+    using H = typename FieldInfo<FID>::Host;
+    using tw = typename FieldInfo<FID>::Dev::tw_t;
+    std::vector<uint32_t> key(g.im_rep, g.im_rep + H::LIMBS);
+    std::lock_guard<std::mutex> lk(g_coset_mutex);
+    auto &slot = g_coset[std::make_tuple(FID, log_n, inverse, key)];
+    if (!slot) {
+        std::unique_ptr<CosetTables> t(new CosetTables());
+        const H base = inverse ? g.mul_inv() : g;
+        t->h = (log_n + 1) / 2;
+        const size_t nlo = (size_t)1 << t->h, nhi = std::max<size_t>(1, ((size_t)1 << log_n) >> t->h);
+        std::vector<tw> lo(nlo), hi(nhi);
+        H acc = H::one();
+        for (size_t x = 0; x < nlo; ++x) {
+            lo[x] = record<FID>(acc);
+            acc *= base;
+        }
+        const H step = acc;
+        acc = H::one();
+        for (size_t y = 0; y < nhi; ++y) {
+            hi[y] = record<FID>(acc);
+            acc *= step;
+        }
+        GSN_TRY(t->lo.alloc(nlo * sizeof(tw)));
+        GSN_TRY(t->hi.alloc(nhi * sizeof(tw)));
+        GSN_TRY(hip_status(hipMemcpy(t->lo.ptr, lo.data(), nlo * sizeof(tw), hipMemcpyHostToDevice), "hipMemcpy"));
+        GSN_TRY(hip_status(hipMemcpy(t->hi.ptr, hi.data(), nhi * sizeof(tw), hipMemcpyHostToDevice), "hipMemcpy"));
+        slot = std::move(t);
+    }
+    *out = slot.get();
+    return GSN_OK;
+}
+
+template <int FID>
+int scale_t(void *out, const void *in, size_t n, const typename FieldInfo<FID>::Host &s, hipStream_t st)
+{
+    using F = typename FieldInfo<FID>::Dev;
+    hipLaunchKernelGGL(ew_scale_kernel<F>, dim3(blocks_for(n)), dim3(256), 0, st, (typename F::elem_t *)out,
+                       (const typename F::elem_t *)in, record<FID>(s), n);
+    return hip_status(hipGetLastError(), "scale launch");
+}
+
+template <int FID>
+int coset_t(void *out, const void *in, size_t n, const void *g_limbs, int inverse, hipStream_t st)
+{
+    using H = typename FieldInfo<FID>::Host;
+    using F = typename FieldInfo<FID>::Dev;
+    unsigned log_n = 0;
+    GSN_TRY(log2_exact(n, &log_n));
+    const H g = H::from_raw((const uint32_t *)g_limbs);
+    if (fields::detail::geq<H::LIMBS>(g.im_rep, H::mod()) || g == H::zero())
+        return set_error(GSN_ERR_INVALID_ARGUMENT, "the coset generator must be a nonzero canonical element");
+    CosetTables *t = nullptr;
+    GSN_TRY(coset_tables<FID>(&t, g, log_n, inverse));
+    hipLaunchKernelGGL(ew_coset_kernel<F>, dim3(blocks_for(n)), dim3(256), 0, st, (typename F::elem_t *)out,
+                       (const typename F::elem_t *)in, (const typename F::tw_t *)t->lo.ptr,
+                       (const typename F::tw_t *)t->hi.ptr, t->h, n);
+    return hip_status(hipGetLastError(), "coset launch");
+}
+
+int check_args(int field, const void *out, const void *in, size_t n)
+{
+    if (!element_bytes(field))
+        return set_error(GSN_ERR_INVALID_ARGUMENT, "unknown field id");
+    if (!out || !in)
+        return set_error(GSN_ERR_INVALID_ARGUMENT, "NULL buffer");
+    if (n == 0)
+        return set_error(GSN_ERR_INVALID_ARGUMENT, "n is 0");
+    return GSN_OK;
+}
+
+}  // namespace
+
+}  // namespace gsn
+
+using namespace gsn;
+
+extern "C" {
+
+int gsn_pointwise_mul(int field, void *d_out, const void *d_a, const void *d_b, size_t n, void *stream)
+{
+    GSN_TRY(check_args(field, d_out, d_a, n));
+    if (!d_b)
+        return set_error(GSN_ERR_INVALID_ARGUMENT, "NULL buffer");
+    GSN_TRY(take_injected_fault());
+    GSN_TRY(ensure_device());
+    hipStream_t st = (hipStream_t)stream;
+    switch (field) {
+    case GSN_FIELD_FP32:
+        hipLaunchKernelGGL(ew_pointwise32_kernel, dim3(blocks_for(n)), dim3(256), 0, st, (uint32_t *)d_out,
+                           (const uint32_t *)d_a, (const uint32_t *)d_b, n);
+        break;
+    case GSN_FIELD_MNT4753_FR:
+        hipLaunchKernelGGL(ew_pointwise753_kernel<Mnt4753FrParams>, dim3(blocks_for(n)), dim3(256), 0, st,
+                           (E753 *)d_out, (const E753 *)d_a, (const E753 *)d_b,
+                           record<GSN_FIELD_MNT4753_FR>(value_r<GSN_FIELD_MNT4753_FR>().mul_inv()), n);
+        break;
+    default:
+        hipLaunchKernelGGL(ew_pointwise753_kernel<Mnt6753FrParams>, dim3(blocks_for(n)), dim3(256), 0, st,
+                           (E753 *)d_out, (const E753 *)d_a, (const E753 *)d_b,
+                           record<GSN_FIELD_MNT6753_FR>(value_r<GSN_FIELD_MNT6753_FR>().mul_inv()), n);
+        break;
+    }
+    return hip_status(hipGetLastError(), "pointwise launch");
+}
+
+int gsn_scale(int field, void *d_out, const void *d_in, size_t n, const void *s, void *stream)
+{
+    GSN_TRY(check_args(field, d_out, d_in, n));
+    if (!s)
+        return set_error(GSN_ERR_INVALID_ARGUMENT, "NULL scale");
+    GSN_TRY(take_injected_fault());
+    GSN_TRY(ensure_device());
+    hipStream_t st = (hipStream_t)stream;
+    switch (field) {
+    case GSN_FIELD_FP32:
+        return scale_t<GSN_FIELD_FP32>(d_out, d_in, n, fields::Fp32::from_raw((const uint32_t *)s), st);
+    case GSN_FIELD_MNT4753_FR:
+        return scale_t<GSN_FIELD_MNT4753_FR>(d_out, d_in, n, fields::Scalar::from_raw((const uint32_t *)s), st);
+    default:
+        return scale_t<GSN_FIELD_MNT6753_FR>(d_out, d_in, n, fields::ScalarMNT6::from_raw((const uint32_t *)s), st);
+    }
+}
+
+int gsn_coset_shift(int field, void *d_out, const void *d_in, size_t n, const void *g, int inverse, void *stream)
+{
+    GSN_TRY(check_args(field, d_out, d_in, n));
+    if (!g)
+        return set_error(GSN_ERR_INVALID_ARGUMENT, "NULL generator");
+    GSN_TRY(take_injected_fault());
+    GSN_TRY(ensure_device());
+    hipStream_t st = (hipStream_t)stream;
+    switch (field) {
+    case GSN_FIELD_FP32: return coset_t<GSN_FIELD_FP32>(d_out, d_in, n, g, inverse, st);
+    case GSN_FIELD_MNT4753_FR: return coset_t<GSN_FIELD_MNT4753_FR>(d_out, d_in, n, g, inverse, st);
+    default: return coset_t<GSN_FIELD_MNT6753_FR>(d_out, d_in, n, g, inverse, st);
+    }
+}
+
+int gsn_to_montgomery(int field, void *d_out, const void *d_in, size_t n, void *stream)
+{
+    GSN_TRY(check_args(field, d_out, d_in, n));
+    GSN_TRY(take_injected_fault());
+    GSN_TRY(ensure_device());
+    hipStream_t st = (hipStream_t)stream;
+    switch (field) {
+    case GSN_FIELD_FP32: return scale_t<GSN_FIELD_FP32>(d_out, d_in, n, value_r<GSN_FIELD_FP32>(), st);
+    case GSN_FIELD_MNT4753_FR:
+        return scale_t<GSN_FIELD_MNT4753_FR>(d_out, d_in, n, value_r<GSN_FIELD_MNT4753_FR>(), st);
+    default: return scale_t<GSN_FIELD_MNT6753_FR>(d_out, d_in, n, value_r<GSN_FIELD_MNT6753_FR>(), st);
+    }
+}
+
+int gsn_from_montgomery(int field, void *d_out, const void *d_in, size_t n, void *stream)
+{
+    GSN_TRY(check_args(field, d_out, d_in, n));
+    GSN_TRY(take_injected_fault());
+    GSN_TRY(ensure_device());
+    hipStream_t st = (hipStream_t)stream;
+    switch (field) {
+    case GSN_FIELD_FP32: return scale_t<GSN_FIELD_FP32>(d_out, d_in, n, value_r<GSN_FIELD_FP32>().mul_inv(), st);
+    case GSN_FIELD_MNT4753_FR:
+        return scale_t<GSN_FIELD_MNT4753_FR>(d_out, d_in, n, value_r<GSN_FIELD_MNT4753_FR>().mul_inv(), st);
+    default:
+        return scale_t<GSN_FIELD_MNT6753_FR>(d_out, d_in, n, value_r<GSN_FIELD_MNT6753_FR>().mul_inv(), st);
+    }
+}
+
+}  // extern "C"

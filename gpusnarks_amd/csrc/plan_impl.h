@@ -215,4 +215,12 @@ struct PlanImpl final : PlanBase {
     }
 };
 
+// twiddle records of field elements (defined in gsn_runtime.hip)
+template <>
+F32::tw_t PlanImpl<GSN_FIELD_FP32>::make_tw(const fields::Fp32 &w);
+template <>
+TW753 PlanImpl<GSN_FIELD_MNT4753_FR>::make_tw(const fields::Scalar &w);
+template <>
+TW753 PlanImpl<GSN_FIELD_MNT6753_FR>::make_tw(const fields::ScalarMNT6 &w);
+
 }  // namespace gsn

@@ -113,6 +113,23 @@ int gsn_debug_clock_ghz(double *ghz);
  * n^-1 scaling), which the reference does not provide. */
 int gsn_fft_host(int field, void *data, size_t n, const void *omega, int inverse);
 
+/* --- domain helpers: the elementwise steps around a transform ----------
+ * None has a counterpart in the reference (its FFT, cuda/fft_kernel.cu:117-147,
+ * is the only operation on the path).  Device buffers of n elements (d_out may
+ * equal the input), asynchronous on `stream`, canonical outputs.  Elements and
+ * constants are in Montgomery form except where a name says otherwise. */
+/* d_out[i] = d_a[i] * d_b[i] (evaluation-form product). */
+int gsn_pointwise_mul(int field, void *d_out, const void *d_a, const void *d_b, size_t n, void *stream);
+/* d_out[i] = d_in[i] * s, s one host element (e.g. n^-1). */
+int gsn_scale(int field, void *d_out, const void *d_in, size_t n, const void *s, void *stream);
+/* d_out[i] = d_in[i] * g^i (inverse != 0: g^-i), n a power of two, g a nonzero
+ * host element: before a forward transform it evaluates on the coset g*H;
+ * after an inverse transform with inverse != 0 it undoes the shift. */
+int gsn_coset_shift(int field, void *d_out, const void *d_in, size_t n, const void *g, int inverse, void *stream);
+/* Plain residues in [0, p) -> Montgomery form (x * R mod p), and back. */
+int gsn_to_montgomery(int field, void *d_out, const void *d_in, size_t n, void *stream);
+int gsn_from_montgomery(int field, void *d_out, const void *d_in, size_t n, void *stream);
+
 /* --- plans: device-resident twiddles, asynchronous execution ------------ */
 typedef struct gsn_plan gsn_plan;
 int gsn_plan_create(gsn_plan **plan, int field, unsigned log_n, const void *omega, int inverse);

@@ -1,0 +1,122 @@
+"""GPU parity of the domain helpers (pointwise product, scaling, coset shift,
+Montgomery conversions) against the CPU oracle's field operations, bit-exact,
+and the coset transform's round trip at full size."""
+import sys
+
+import numpy as np
+import pytest
+
+import gpusnarks_amd as g
+from gpusnarks_amd import synth
+from tests.oracle_lib import Oracle
+
+pytestmark = pytest.mark.gpu
+FIELDS = [g.FIELD_FP32, g.FIELD_MNT4753_FR, g.FIELD_MNT6753_FR]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return Oracle()
+
+
+def residues(field, n, seed):
+    x = synth.uniform_residues(seed, n, g.MODULI[field], g.NLIMBS[field])
+    # edge values: 0, 1, p - 1, R mod p (Montgomery one)
+    p = g.MODULI[field]
+    L = g.NLIMBS[field]
+    edge = [0, 1, p - 1, (1 << (32 * L)) % p]
+    for i, v in enumerate(edge):
+        x[i] = [(v >> (32 * k)) & 0xFFFFFFFF for k in range(L)]
+    return x
+
+
+@pytest.mark.parametrize("field", FIELDS)
+def test_pointwise_and_scale(orc, field):
+    n = 1000  # any n: elementwise
+    a, b = residues(field, n, 0xA1 + field), residues(field, n, 0xB2 + field)
+    say("pointwise/scale field", field)
+    da, db, dout = g.DeviceArray.from_numpy(a), g.DeviceArray.from_numpy(b), g.DeviceArray(a.nbytes)
+    g.pointwise_mul(field, dout.ptr, da.ptr, db.ptr, n)
+    g.synchronize()
+    assert np.array_equal(dout.download(a.shape), orc.mul(field, a, b))
+    s = b[7]
+    g.scale(field, dout.ptr, da.ptr, n, s)
+    g.synchronize()
+    assert np.array_equal(dout.download(a.shape), orc.mul(field, a, np.repeat(s[None], n, 0)))
+    # in place
+    g.pointwise_mul(field, da.ptr, da.ptr, db.ptr, n)
+    g.synchronize()
+    assert np.array_equal(da.download(a.shape), orc.mul(field, a, b))
+
+
+@pytest.mark.parametrize("field", FIELDS)
+def test_montgomery_conversions(orc, field):
+    n = 777
+    x = residues(field, n, 0xC3 + field)  # read as plain residues
+    say("montgomery conversions field", field)
+    prm = orc.params(field)
+    dx, dy = g.DeviceArray.from_numpy(x), g.DeviceArray(x.nbytes)
+    g.to_montgomery(field, dy.ptr, dx.ptr, n)
+    g.synchronize()
+    mont = dy.download(x.shape)
+    # x R = mont(x, R^2) with the oracle's Montgomery product
+    assert np.array_equal(mont, orc.mul(field, x, np.repeat(prm["r2"][None], n, 0)))
+    g.from_montgomery(field, dx.ptr, dy.ptr, n)
+    g.synchronize()
+    assert np.array_equal(dx.download(x.shape), x)
+
+
+@pytest.mark.parametrize("field,log_n", [(g.FIELD_FP32, 10), (g.FIELD_MNT4753_FR, 9), (g.FIELD_MNT6753_FR, 9)])
+def test_coset_shift_matches_powers(orc, field, log_n):
+    n = 1 << log_n
+    x = residues(field, n, 0xD4 + field)
+    gen = residues(field, 8, 0xE5 + field)[5]
+    say("coset shift field", field, "n", n)
+    powers = orc.pow(field, np.repeat(gen[None], n, 0), list(range(n)))
+    dx, dy = g.DeviceArray.from_numpy(x), g.DeviceArray(x.nbytes)
+    g.coset_shift(field, dy.ptr, dx.ptr, n, gen)
+    g.synchronize()
+    shifted = dy.download(x.shape)
+    assert np.array_equal(shifted, orc.mul(field, x, powers))
+    g.coset_shift(field, dy.ptr, dy.ptr, n, gen, inverse=True)
+    g.synchronize()
+    assert np.array_equal(dy.download(x.shape), x)
+
+
+@pytest.mark.parametrize("field,log_n", [(g.FIELD_FP32, 20), (g.FIELD_MNT6753_FR, 16)])
+def test_coset_transform_round_trip(field, log_n):
+    """Evaluate on the coset g*H (shift, forward NTT), interpolate back
+    (inverse NTT, unshift): the identity, at full size."""
+    n = 1 << log_n
+    x = synth.uniform_residues(0xF6 + log_n, n, g.MODULI[field], g.NLIMBS[field])
+    gen = synth.uniform_residues(0x77, 1, g.MODULI[field], g.NLIMBS[field])[0]  # a random nonzero shift
+    say("coset round trip field", field, "n", n)
+    d = g.DeviceArray.from_numpy(x)
+    fwd, inv = g.Plan(field, log_n), g.Plan(field, log_n, inverse=True)
+    s = g.Stream()
+    g.coset_shift(field, d.ptr, d.ptr, n, gen, stream=s.handle)
+    fwd.execute(d.ptr, d.ptr, s.handle)
+    s.synchronize()
+    evals = d.download(x.shape)
+    assert not np.array_equal(evals, x)
+    inv.execute(d.ptr, d.ptr, s.handle)
+    g.coset_shift(field, d.ptr, d.ptr, n, gen, inverse=True, stream=s.handle)
+    s.synchronize()
+    assert np.array_equal(d.download(x.shape), x)
+
+
+def test_domain_argument_errors():
+    d = g.DeviceArray(4096)
+    with pytest.raises(g.GsnError):
+        g.coset_shift(g.FIELD_FP32, d.ptr, d.ptr, 1000, np.array([3], np.uint32))  # n not a power of two
+    with pytest.raises(g.GsnError):
+        g.coset_shift(g.FIELD_FP32, d.ptr, d.ptr, 1024, np.array([0], np.uint32))  # g = 0
+    with pytest.raises(g.GsnError):
+        g.pointwise_mul(7, d.ptr, d.ptr, d.ptr, 16)
+    with pytest.raises(g.GsnError):
+        g.scale(g.FIELD_FP32, d.ptr, d.ptr, 0, np.array([1], np.uint32))
