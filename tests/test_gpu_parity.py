@@ -133,3 +133,32 @@ def test_cpp_drop_in_gpu(tmp_path):
     r = subprocess.run([exe, "gpu"], capture_output=True, text=True, timeout=100)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "gpu: 0 failure(s)" in r.stdout
+
+
+SCHEDULES = [
+    (g.FIELD_MNT6753_FR, 12, [[(6, 1, 1), (6, 1, 1)], [(7, 1, 1), (5, 2, 1)], [(8, 1, 1), (4, 4, 1)],
+                              [(9, 1, 1), (3, 7, 1)], [(10, 0, 1), (2, 8, 1)], [(4, 3, 1)] * 3, [(2, 5, 1)] * 6]),
+    (g.FIELD_MNT4753_FR, 13, [[(9, 0, 1), (4, 5, 1)], [(5, 5, 1), (8, 2, 1)]]),
+    (g.FIELD_FP32, 16, [[(8, 4, 4), (8, 4, 4)], [(8, 5, 4), (8, 5, 4)], [(6, 5, 4), (5, 5, 4), (5, 5, 4)],
+                        [(9, 4, 4), (7, 4, 4)], [(10, 4, 4), (6, 4, 4)]]),
+]
+
+
+@pytest.mark.parametrize("field,log_n,schedules", SCHEDULES)
+def test_every_tile_shape_matches_oracle(orc, field, log_n, schedules):
+    """Explicit pass schedules covering every 753-bit workgroup size (64..512
+    threads), wide and one-column tiles, and the 16- and 32-column 32-bit
+    shapes: each bit-exact against the oracle, forward and inverse."""
+    x = inputs(field, log_n, 0x5C4ED + log_n)
+    om = g.root_of_unity(field, log_n)
+    want = orc.serial_fft(field, x, om)
+    before = g.tuning_get(field, log_n)
+    try:
+        for sched in schedules:
+            say("schedule", NAMES[field], log_n, sched)
+            g.tuning_set(field, log_n, sched)
+            got = g.best_fft(x, om, field)
+            assert np.array_equal(got, want), sched
+            assert np.array_equal(g.best_ifft(got, om, field), x), sched
+    finally:
+        g.tuning_set(field, log_n, before)
