@@ -599,6 +599,21 @@ size_t gsn_plan_device_bytes(const gsn_plan *plan) { return plan ? plan->impl->d
 
 int gsn_plan_num_passes(const gsn_plan *plan) { return plan ? plan->impl->num_passes : 0; }
 
+// Every element below p (the kernels' value bounds assume canonical inputs):
+// the top limb decides unless it equals p's.
+extern "C++" template <int L>
+static int check_canonical(const uint32_t *x, size_t n, const uint32_t *mod)
+{
+    for (size_t i = 0; i < n; ++i) {
+        const uint32_t *e = x + i * L;
+        if (e[L - 1] < mod[L - 1])
+            continue;
+        if (e[L - 1] > mod[L - 1] || fields::detail::geq<L>(e, mod))
+            return set_error(GSN_ERR_INVALID_ARGUMENT, "element " + std::to_string(i) + " is not below p");
+    }
+    return GSN_OK;
+}
+
 int gsn_fft_host(int field, void *data, size_t n, const void *omega, int inverse)
 {
     if (!data)
@@ -608,6 +623,13 @@ int gsn_fft_host(int field, void *data, size_t n, const void *omega, int inverse
         return set_error(GSN_ERR_INVALID_ARGUMENT, "unknown field id");
     unsigned log_n = 0;
     GSN_TRY(log2_exact(n, &log_n));
+    switch (field) {
+    case GSN_FIELD_FP32: GSN_TRY(check_canonical<1>((const uint32_t *)data, n, &Fp32Params::P)); break;
+    case GSN_FIELD_MNT4753_FR:
+        GSN_TRY(check_canonical<24>((const uint32_t *)data, n, Mnt4753FrParams::MOD));
+        break;
+    default: GSN_TRY(check_canonical<24>((const uint32_t *)data, n, Mnt6753FrParams::MOD)); break;
+    }
     PlanBase *p = nullptr;
     GSN_TRY(cached_plan(&p, field, log_n, omega, inverse));
     std::lock_guard<std::mutex> lk(p->host_mutex);

@@ -112,6 +112,11 @@ int gsn_debug_clock_ghz(double *ghz);
  * synchronous.  inverse != 0 computes the inverse transform (omega^-1 and
  * n^-1 scaling), which the reference does not provide. */
 int gsn_fft_host(int field, void *data, size_t n, const void *omega, int inverse);
+/* Elements must be canonical (below p) on every transform entry point: the
+ * kernels keep intermediate values partly reduced and their bounds start
+ * from inputs below p.  gsn_fft_host checks this (GSN_ERR_INVALID_ARGUMENT);
+ * the plan, batch and server entry points do not, and give unspecified
+ * (memory-safe) results for inputs at or above p. */
 
 /* --- domain helpers: the elementwise steps around a transform ----------
  * None has a counterpart in the reference (its FFT, cuda/fft_kernel.cu:117-147,

@@ -111,3 +111,15 @@ def test_quotient_estimation_constants(struct):
         assert ((((1 << 783) - rem) * pinv) % (1 << 783)) == (w << 783) // p
     # the value bound the kernels rely on: 91 p < 2^761 (30 stages of +3p)
     assert 91 * p < (1 << 761) and 3 * p < (1 << 755)
+
+
+def test_host_entry_rejects_non_canonical_input():
+    """gsn_fft_host refuses elements at or above p before any device work."""
+    for field in (g.FIELD_FP32, g.FIELD_MNT6753_FR):
+        p = g.MODULI[field]
+        L = g.NLIMBS[field]
+        x = np.zeros((8, L), np.uint32)
+        x[3] = [(p >> (32 * k)) & 0xFFFFFFFF for k in range(L)]  # exactly p
+        with pytest.raises(g.GsnError) as e:
+            g.best_fft(x, None, field)
+        assert e.value.status == 1 and "not below p" in g.lib().gsn_last_error().decode()
