@@ -294,9 +294,13 @@ class Server {
             std::unique_lock<std::mutex> lk(m_);
             if (stop_ && pending_.empty() && waves_.empty())
                 break;
-            if (!waves_.empty())
-                cv_.wait_for(lk, std::chrono::microseconds(20));
-            else if (pending_.empty())
+            if (!waves_.empty()) {
+                // waves in flight: poll their events without sleeping (a
+                // timed wait would round up to the kernel's timer slack,
+                // tens of microseconds added to every request's latency)
+                lk.unlock();
+                std::this_thread::yield();
+            } else if (pending_.empty())
                 cv_.wait(lk, [this] { return stop_ || !pending_.empty(); });
             else
                 cv_.wait_for(lk, std::chrono::microseconds(200));  // waiting for memory
