@@ -15,7 +15,10 @@ HBM), one transform per step.  The same line carries:
   serving       Poisson arrivals (70% 753-bit / 30% 32-bit, log2 n in
                 [12, 20]) through the continuous-batching server against
                 one-at-a-time plain dispatch: p50/p99 latency, throughput;
-  fp32          the 32-bit field at n = 2^26 against HBM bandwidth.
+  fp32          the 32-bit field at n = 2^26 against HBM bandwidth;
+  configs       every configured size as a plain transform;
+  per_transform median/min/max of individually timed transforms;
+  end_to_end    the host-buffer entry point (best_fft) including its copies.
 
     python bench.py [--gpus 1] [--steps 50] [--warmup 5]
 """
@@ -82,6 +85,60 @@ def time_plan(field, log_n, steps, warmup, inverse=False):
     return ms / steps, plan.num_passes
 
 
+def step_times(field, log_n, runs=20, warmup=3):
+    """Per-transform durations (HIP events around each execute, operands in
+    HBM): median and min over `runs` transforms."""
+    import gpusnarks_amd as g
+    x = inputs(field, log_n, 0x5EED0000 + log_n)
+    plan = g.Plan(field, log_n)
+    d_in = g.DeviceArray.from_numpy(x)
+    d_out = g.DeviceArray(x.nbytes)
+    s = g.Stream()
+    for _ in range(warmup):
+        plan.execute(d_in.ptr, d_out.ptr, s.handle)
+    ev = [g.Event() for _ in range(runs + 1)]
+    ev[0].record(s.handle)
+    for i in range(runs):
+        plan.execute(d_in.ptr, d_out.ptr, s.handle)
+        ev[i + 1].record(s.handle)
+    ev[-1].synchronize()
+    t = sorted(g.Event.elapsed_ms(ev[i], ev[i + 1]) for i in range(runs))
+    return {"runs": runs, "median_ms": t[runs // 2], "min_ms": t[0], "max_ms": t[-1]}
+
+
+def end_to_end(field, log_n, runs=5):
+    """The host-buffer entry point (gsn_fft_host, the best_fft call): input
+    copied to HBM, transformed, copied back; wall clock, median of `runs`."""
+    import gpusnarks_amd as g
+    x = inputs(field, log_n, 0x5EED0100 + log_n)
+    L = g.lib()
+    ptr = x.ctypes.data_as(ctypes.c_void_p)
+    g.check(L.gsn_fft_host(field, ptr, x.shape[0], None, 0))  # plan, twiddles, staging buffers
+    t = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        g.check(L.gsn_fft_host(field, ptr, x.shape[0], None, 0))
+        t.append((time.perf_counter() - t0) * 1e3)
+    t.sort()
+    ms = t[runs // 2]
+    return {"runs": runs, "median_ms": ms, "bytes_each_way": int(x.nbytes),
+            "butterflies_per_s": bfly(log_n) / (ms * 1e-3),
+            "note": "pageable host buffer in, transform, host buffer out (PCIe-inclusive); not the headline"}
+
+
+def cpu_info():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0))}
+
+
 def cpu_baseline(log_n, threads, field=None, serial=True):
     """Oracle (test/fft_host.h restated) on the host: the reference
     harness's parallel radix-2 FFT with log_cpus = 10, bounded sample."""
@@ -120,6 +177,7 @@ def cpu_baseline(log_n, threads, field=None, serial=True):
                                                              threads),
         "seconds_extrapolated": total,
     }
+    out.update(cpu_info())
     if serial:
         # and the O(n log n) serial transform on one core, in full
         t0 = time.perf_counter()
@@ -388,6 +446,7 @@ def main():
     roofline["clock_measured_ghz"] = ghz.value
     roofline["peak_at_measured_clock"] = MAD_PEAK / 1e12 * ghz.value * 1e9 / CLOCK_HZ
     roofline["frac_at_measured_clock"] = achieved / (MAD_PEAK * ghz.value * 1e9 / CLOCK_HZ)
+    line_timing = step_times(field, log_n)
     traffic_file = os.path.join(ROOT, "profiles", "ntt753_n18_traffic.json")
     if os.path.exists(traffic_file):
         with open(traffic_file) as fh:
@@ -402,6 +461,7 @@ def main():
                                "operands resident in HBM, one transform per step" % log_n,
                    "field": "MNT6-753 Fr", "log2_n": log_n, "elements_per_s": (1 << log_n) / (ms * 1e-3)},
         "roofline": roofline,
+        "per_transform": line_timing,
     }
     if not args.skip_cpu:
         threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
@@ -416,7 +476,11 @@ def main():
             "algorithmic_hbm_bytes": bytes_alg, "achieved_GBps": bytes_alg / (ms32 * 1e-3) / 1e9,
             "frac_of_8TBps": bytes_alg / (ms32 * 1e-3) / HBM_PEAK,
             "actual_pass_bytes_GBps": 8 * (1 << 26) * p32 / (ms32 * 1e-3) / 1e9,
+            "per_transform": step_times(g.FIELD_FP32, 26),
         }
+        log("end to end (host buffers)")
+        line["end_to_end"] = {"mnt6_n18": end_to_end(g.FIELD_MNT6753_FR, log_n),
+                              "fp32_n26": end_to_end(g.FIELD_FP32, 26)}
         log("batched ragged workload")
         line["batched"] = bench_batched()
         log("batched 16 x 2^%d" % log_n)

@@ -102,6 +102,9 @@ def lib():
     L.gsn_debug_copy_u32.argtypes = [vp, vp, sz, vp]
     L.gsn_debug_clock_ghz.argtypes = [ctypes.POINTER(ctypes.c_double)]
     L.gsn_pointwise_mul.argtypes = [ctypes.c_int, vp, vp, vp, sz, vp]
+    L.gsn_vec_add.argtypes = [ctypes.c_int, vp, vp, vp, sz, vp]
+    L.gsn_vec_sub.argtypes = [ctypes.c_int, vp, vp, vp, sz, vp]
+    L.gsn_vec_pow.argtypes = [ctypes.c_int, vp, vp, sz, ctypes.c_uint64, vp]
     L.gsn_scale.argtypes = [ctypes.c_int, vp, vp, sz, vp, vp]
     L.gsn_coset_shift.argtypes = [ctypes.c_int, vp, vp, sz, vp, ctypes.c_int, vp]
     L.gsn_to_montgomery.argtypes = [ctypes.c_int, vp, vp, sz, vp]
@@ -178,6 +181,23 @@ def pointwise_mul(field, d_out, d_a, d_b, n, stream=None):
     """d_out[i] = d_a[i] * d_b[i] (Montgomery forms)."""
     check(lib().gsn_pointwise_mul(field, ctypes.c_void_p(d_out), ctypes.c_void_p(d_a), ctypes.c_void_p(d_b), n,
                                   stream))
+
+
+def vec_add(field, d_out, d_a, d_b, n, stream=None):
+    """d_out[i] = d_a[i] + d_b[i] (Scalar operator+, cuda/device_field.h:124-131)."""
+    check(lib().gsn_vec_add(field, ctypes.c_void_p(d_out), ctypes.c_void_p(d_a), ctypes.c_void_p(d_b), n,
+                            stream))
+
+
+def vec_sub(field, d_out, d_a, d_b, n, stream=None):
+    """d_out[i] = d_a[i] - d_b[i] (Scalar operator-, cuda/device_field.h:133-140)."""
+    check(lib().gsn_vec_sub(field, ctypes.c_void_p(d_out), ctypes.c_void_p(d_a), ctypes.c_void_p(d_b), n,
+                            stream))
+
+
+def vec_pow(field, d_out, d_in, n, exponent, stream=None):
+    """d_out[i] = d_in[i] ** exponent (Scalar operator^, cuda/device_field.h:151-158)."""
+    check(lib().gsn_vec_pow(field, ctypes.c_void_p(d_out), ctypes.c_void_p(d_in), n, int(exponent), stream))
 
 
 def scale(field, d_out, d_in, n, s, stream=None):

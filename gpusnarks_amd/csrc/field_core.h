@@ -196,6 +196,90 @@ struct F753 {
         }
     }
 
+    // general product of two elements in Montgomery form (R = 2^768):
+    // a b 2^-768 mod p, canonical, for canonical a and b.  REDC in 29-bit
+    // digits with radix 2^783 on a' = a 2^15 (a' b 2^-783 = a b 2^-768),
+    // column by column: column c < 27 fixes the reduction digit m_c that
+    // clears it; columns 27..53 are the result, a' b + m p < 2^783 2p.
+    // Each column holds at most 53 products of 58 bits: 64-bit sums.
+    __device__ __forceinline__ static E753 mont_mul(const E753 &a, const E753 &b)
+    {
+        uint32_t x[27], y[27], m[27], r[27];
+#pragma unroll
+        for (int i = 0; i < 27; ++i) {
+            // digit i of a 2^15: bits [29i - 15, 29i + 14) of a
+            const int bit = 29 * i - 15;
+            if (bit < 0) {
+                x[i] = (a.v[0] << 15) & M29;
+            } else {
+                const int w = bit >> 5, sh = bit & 31;
+                const uint32_t lo = a.v[w];
+                const uint32_t hi = (w + 1 < 24) ? a.v[w + 1] : 0u;
+                x[i] = (sh ? ((lo >> sh) | (hi << (32 - sh))) : lo) & M29;
+            }
+        }
+        to29(y, b.v);
+        constexpr uint32_t MP29 = P::MPRIME & M29;  // -p^-1 mod 2^29
+        uint64_t acc = 0;
+#pragma unroll
+        for (int c = 0; c < 27; ++c) {
+#pragma unroll
+            for (int i = 0; i <= c; ++i)
+                acc += (uint64_t)x[i] * y[c - i];
+#pragma unroll
+            for (int i = (c > 25 ? c - 25 : 0); i < c; ++i)
+                acc += (uint64_t)m[i] * P::MOD29[c - i];
+            m[c] = ((uint32_t)acc * MP29) & M29;
+            acc += (uint64_t)m[c] * P::MOD29[0];
+            acc >>= 29;
+        }
+#pragma unroll
+        for (int c = 27; c < 54; ++c) {
+#pragma unroll
+            for (int i = c - 26; i < 27; ++i)
+                acc += (uint64_t)x[i] * y[c - i];
+#pragma unroll
+            for (int i = c - 25; i < 27; ++i)
+                acc += (uint64_t)m[i] * P::MOD29[c - i];
+            r[c - 27] = (uint32_t)acc & M29;
+            acc >>= 29;
+        }
+        E753 t;
+        from29(t.v, r);
+        return sub_p_if_geq(t);
+    }
+    // x - p if x >= p, for x < 2p
+    __device__ __forceinline__ static E753 sub_p_if_geq(const E753 &x)
+    {
+        uint32_t d[24];
+        unsigned bw = 0;
+#pragma unroll
+        for (int i = 0; i < 24; ++i)
+            d[i] = __builtin_subc(x.v[i], P::MOD[i], bw, &bw);
+        E753 o;
+#pragma unroll
+        for (int i = 0; i < 24; ++i)
+            o.v[i] = bw ? x.v[i] : d[i];
+        return o;
+    }
+    // (a + b) mod p and (a - b) mod p for canonical a, b
+    __device__ __forceinline__ static E753 add_mod(const E753 &a, const E753 &b) { return sub_p_if_geq(add(a, b)); }
+    __device__ __forceinline__ static E753 sub_mod(const E753 &a, const E753 &b)
+    {
+        E753 d;
+        unsigned bw = 0;
+#pragma unroll
+        for (int i = 0; i < 24; ++i)
+            d.v[i] = __builtin_subc(a.v[i], b.v[i], bw, &bw);
+        if (bw) {
+            unsigned c = 0;
+#pragma unroll
+            for (int i = 0; i < 24; ++i)
+                d.v[i] = __builtin_addc(d.v[i], P::MOD[i], c, &c);
+        }
+        return d;
+    }
+
     // y * w mod p as a value in [0, 3p), for any y < 2^768
     __device__ __forceinline__ static E753 mul(const E753 &y, const TW753 &t)
     {

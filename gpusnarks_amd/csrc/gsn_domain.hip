@@ -1,18 +1,25 @@
 // Domain helpers around the transform: the elementwise steps that sit on
 // either side of an NTT in a prover (the reference has none of them; its
-// FFT, cuda/fft_kernel.cu:117-147, is the only operation on the path):
+// FFT, cuda/fft_kernel.cu:117-147, is the only operation on the path), and
+// the field's operators over whole vectors (the reference's Scalar
+// operator+ / - / * / ^, cuda/device_field.h:115-158, one element at a time):
 //
 //   gsn_pointwise_mul    out[i] = a[i] * b[i]            (evaluation-form product)
+//   gsn_vec_add          out[i] = a[i] + b[i]
+//   gsn_vec_sub          out[i] = a[i] - b[i]
+//   gsn_vec_pow          out[i] = in[i]^e                 (e a 64-bit integer)
 //   gsn_scale            out[i] = in[i] * s               (e.g. n^-1, a constant)
 //   gsn_coset_shift      out[i] = in[i] * g^(+-i)         (coset NTT: shift, transform)
 //   gsn_to_montgomery    out[i] = in[i] * R mod p         (plain residues -> API form)
 //   gsn_from_montgomery  out[i] = in[i] * R^-1 mod p      (API form -> plain residues)
 //
-// All work on device buffers (out may equal the input), are asynchronous on
-// the given stream and write canonical residues.  Every product is the
-// field's twiddle product (F32::mul, F753<P>::mul) by a precomputed record of
-// the constant factor; a coset's g^i is lo[i mod 2^h] * hi[i >> h] from two
-// tables of about sqrt(n) records, cached per (field, g, n, direction).
+// All work on device buffers of canonical residues (out may equal an input),
+// are asynchronous on the given stream and write canonical residues.
+// Products of two vectors are general Montgomery products (F753<P>::mont_mul);
+// products by a constant are the field's twiddle product (F32::mul,
+// F753<P>::mul) by a precomputed record of the constant; a coset's g^i is
+// lo[i mod 2^h] * hi[i >> h] from two tables of about sqrt(n) records, cached
+// per (field, g, n, direction).
 #include <hip/hip_runtime.h>
 
 #include <map>
@@ -52,34 +59,69 @@ __global__ __launch_bounds__(256) void ew_coset_kernel(typename F::elem_t *out, 
     F::store(out, i, F::canon(F::mul(v, F::load_tw(hi, i >> h))));
 }
 
-// out[i] = a[i] * b[i]: b[i] becomes a twiddle record first
-__device__ __forceinline__ uint32_t pointwise_one(uint32_t a, uint32_t b)
-{
-    return F32::mul(a, F32::tw_t{b, b * Fp32Params::PINV});
-}
+// Elementwise field operations on canonical operands, one adapter per field:
+// add/sub/mul give canonical results (the 32-bit product is the Montgomery
+// product with the multiplier's companion computed on the fly; the 753-bit
+// one is F753<P>::mont_mul).
+struct Ops32 {
+    using elem = uint32_t;
+    __device__ static elem add(elem a, elem b) { return F32::add(a, b); }
+    __device__ static elem sub(elem a, elem b) { return F32::sub(a, b); }
+    __device__ static elem mul(elem a, elem b) { return F32::mul(a, F32::tw_t{b, b * Fp32Params::PINV}); }
+    __device__ static elem one() { return Fp32Params::ONE; }
+    __device__ static elem load(const elem *p, size_t i) { return p[i]; }
+    __device__ static void store(elem *p, size_t i, elem v) { p[i] = v; }
+};
 template <class P>
-__device__ __forceinline__ E753 pointwise_one(const E753 &a, const E753 &b, const TW753 &rinv)
-{
-    // b = b' R (Montgomery form): its value b' = b * R^-1, as a record
-    const TW753 rec = F753<P>::make_tw(F753<P>::canon(F753<P>::mul(b, rinv)));
-    return F753<P>::canon(F753<P>::mul(a, rec));
-}
+struct Ops753 {
+    using elem = E753;
+    using F = F753<P>;
+    __device__ static elem add(const elem &a, const elem &b) { return F::add_mod(a, b); }
+    __device__ static elem sub(const elem &a, const elem &b) { return F::sub_mod(a, b); }
+    __device__ static elem mul(const elem &a, const elem &b) { return F::mont_mul(a, b); }
+    __device__ static elem one()
+    {
+        elem o;
+#pragma unroll
+        for (int i = 0; i < 24; ++i)
+            o.v[i] = P::ONE[i];
+        return o;
+    }
+    __device__ static elem load(const elem *p, size_t i) { return F::load(p, i); }
+    __device__ static void store(elem *p, size_t i, const elem &v) { F::store(p, i, v); }
+};
 
-__global__ __launch_bounds__(256) void ew_pointwise32_kernel(uint32_t *out, const uint32_t *a, const uint32_t *b,
-                                                            size_t n)
+enum { EW_ADD = 0, EW_SUB = 1, EW_MUL = 2 };
+
+// out[i] = a[i] (+ - *) b[i]
+template <class O, int OP>
+__global__ __launch_bounds__(256) void ew_binop_kernel(typename O::elem *out, const typename O::elem *a,
+                                                      const typename O::elem *b, size_t n)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n)
-        out[i] = pointwise_one(a[i], b[i]);
+    if (i >= n)
+        return;
+    const typename O::elem x = O::load(a, i), y = O::load(b, i);
+    O::store(out, i, OP == EW_ADD ? O::add(x, y) : OP == EW_SUB ? O::sub(x, y) : O::mul(x, y));
 }
 
-template <class P>
-__global__ __launch_bounds__(256) void ew_pointwise753_kernel(E753 *out, const E753 *a, const E753 *b, TW753 rinv,
-                                                             size_t n)
+// out[i] = in[i]^e: left-to-right square and multiply over the exponent's
+// bits (uniform across lanes), starting from the field's one
+template <class O>
+__global__ __launch_bounds__(256) void ew_pow_kernel(typename O::elem *out, const typename O::elem *in, size_t n,
+                                                    uint64_t e)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n)
-        F753<P>::store(out, i, pointwise_one<P>(F753<P>::load(a, i), F753<P>::load(b, i), rinv));
+    if (i >= n)
+        return;
+    const typename O::elem x = O::load(in, i);
+    typename O::elem r = O::one();
+    for (int bit = 63 - __builtin_clzll(e | 1); bit >= 0; --bit) {
+        r = O::mul(r, r);
+        if ((e >> bit) & 1u)
+            r = O::mul(r, x);
+    }
+    O::store(out, i, r);
 }
 
 // ------------------------------------------------------------ host side
@@ -181,7 +223,36 @@ int check_args(int field, const void *out, const void *in, size_t n)
         return set_error(GSN_ERR_INVALID_ARGUMENT, "NULL buffer");
     if (n == 0)
         return set_error(GSN_ERR_INVALID_ARGUMENT, "n is 0");
+    if (n > (size_t)0xFFFFFFFFu * 256u)
+        return set_error(GSN_ERR_INVALID_ARGUMENT, "n exceeds one launch's grid");
     return GSN_OK;
+}
+
+template <int OP>
+int binop(int field, void *d_out, const void *d_a, const void *d_b, size_t n, void *stream)
+{
+    GSN_TRY(check_args(field, d_out, d_a, n));
+    if (!d_b)
+        return set_error(GSN_ERR_INVALID_ARGUMENT, "NULL buffer");
+    GSN_TRY(take_injected_fault());
+    GSN_TRY(ensure_device());
+    hipStream_t st = (hipStream_t)stream;
+    const dim3 grid(blocks_for(n)), block(256);
+    switch (field) {
+    case GSN_FIELD_FP32:
+        hipLaunchKernelGGL((ew_binop_kernel<Ops32, OP>), grid, block, 0, st, (uint32_t *)d_out, (const uint32_t *)d_a,
+                           (const uint32_t *)d_b, n);
+        break;
+    case GSN_FIELD_MNT4753_FR:
+        hipLaunchKernelGGL((ew_binop_kernel<Ops753<Mnt4753FrParams>, OP>), grid, block, 0, st, (E753 *)d_out,
+                           (const E753 *)d_a, (const E753 *)d_b, n);
+        break;
+    default:
+        hipLaunchKernelGGL((ew_binop_kernel<Ops753<Mnt6753FrParams>, OP>), grid, block, 0, st, (E753 *)d_out,
+                           (const E753 *)d_a, (const E753 *)d_b, n);
+        break;
+    }
+    return hip_status(hipGetLastError(), "elementwise launch");
 }
 
 }  // namespace
@@ -194,29 +265,41 @@ extern "C" {
 
 int gsn_pointwise_mul(int field, void *d_out, const void *d_a, const void *d_b, size_t n, void *stream)
 {
-    GSN_TRY(check_args(field, d_out, d_a, n));
-    if (!d_b)
-        return set_error(GSN_ERR_INVALID_ARGUMENT, "NULL buffer");
+    return binop<EW_MUL>(field, d_out, d_a, d_b, n, stream);
+}
+
+int gsn_vec_add(int field, void *d_out, const void *d_a, const void *d_b, size_t n, void *stream)
+{
+    return binop<EW_ADD>(field, d_out, d_a, d_b, n, stream);
+}
+
+int gsn_vec_sub(int field, void *d_out, const void *d_a, const void *d_b, size_t n, void *stream)
+{
+    return binop<EW_SUB>(field, d_out, d_a, d_b, n, stream);
+}
+
+int gsn_vec_pow(int field, void *d_out, const void *d_in, size_t n, uint64_t exponent, void *stream)
+{
+    GSN_TRY(check_args(field, d_out, d_in, n));
     GSN_TRY(take_injected_fault());
     GSN_TRY(ensure_device());
     hipStream_t st = (hipStream_t)stream;
+    const dim3 grid(blocks_for(n)), block(256);
     switch (field) {
     case GSN_FIELD_FP32:
-        hipLaunchKernelGGL(ew_pointwise32_kernel, dim3(blocks_for(n)), dim3(256), 0, st, (uint32_t *)d_out,
-                           (const uint32_t *)d_a, (const uint32_t *)d_b, n);
+        hipLaunchKernelGGL(ew_pow_kernel<Ops32>, grid, block, 0, st, (uint32_t *)d_out, (const uint32_t *)d_in, n,
+                           exponent);
         break;
     case GSN_FIELD_MNT4753_FR:
-        hipLaunchKernelGGL(ew_pointwise753_kernel<Mnt4753FrParams>, dim3(blocks_for(n)), dim3(256), 0, st,
-                           (E753 *)d_out, (const E753 *)d_a, (const E753 *)d_b,
-                           record<GSN_FIELD_MNT4753_FR>(value_r<GSN_FIELD_MNT4753_FR>().mul_inv()), n);
+        hipLaunchKernelGGL(ew_pow_kernel<Ops753<Mnt4753FrParams>>, grid, block, 0, st, (E753 *)d_out,
+                           (const E753 *)d_in, n, exponent);
         break;
     default:
-        hipLaunchKernelGGL(ew_pointwise753_kernel<Mnt6753FrParams>, dim3(blocks_for(n)), dim3(256), 0, st,
-                           (E753 *)d_out, (const E753 *)d_a, (const E753 *)d_b,
-                           record<GSN_FIELD_MNT6753_FR>(value_r<GSN_FIELD_MNT6753_FR>().mul_inv()), n);
+        hipLaunchKernelGGL(ew_pow_kernel<Ops753<Mnt6753FrParams>>, grid, block, 0, st, (E753 *)d_out,
+                           (const E753 *)d_in, n, exponent);
         break;
     }
-    return hip_status(hipGetLastError(), "pointwise launch");
+    return hip_status(hipGetLastError(), "pow launch");
 }
 
 int gsn_scale(int field, void *d_out, const void *d_in, size_t n, const void *s, void *stream)

@@ -55,6 +55,13 @@ struct PlanBase {
     virtual int execute(const void *in, void *out, hipStream_t s) = 0;
 };
 
+// Copies of caller-owned host buffers through pinned, double-buffered
+// staging (synchronous with respect to the host buffer: staged_h2d may
+// return before the device holds the data, but work later on `st` sees it;
+// staged_d2h returns once the host buffer is filled).
+int staged_h2d(void *dev, const void *host, size_t bytes, hipStream_t st);
+int staged_d2h(void *host, const void *dev, size_t bytes, hipStream_t st);
+
 int create_plan(PlanBase **out, int field, unsigned log_n, const void *omega, int inverse);
 void clear_schedules();
 void clear_plan_cache();

@@ -634,9 +634,9 @@ int gsn_fft_host(int field, void *data, size_t n, const void *omega, int inverse
     GSN_TRY(cached_plan(&p, field, log_n, omega, inverse));
     std::lock_guard<std::mutex> lk(p->host_mutex);
     GSN_TRY(p->host_buf.alloc(n * eb));
-    GSN_TRY(hip_status(hipMemcpy(p->host_buf.ptr, data, n * eb, hipMemcpyHostToDevice), "hipMemcpy H2D"));
+    GSN_TRY(staged_h2d(p->host_buf.ptr, data, n * eb, nullptr));
     GSN_TRY(p->execute(p->host_buf.ptr, p->host_buf.ptr, nullptr));
-    GSN_TRY(hip_status(hipMemcpy(data, p->host_buf.ptr, n * eb, hipMemcpyDeviceToHost), "hipMemcpy D2H"));
+    GSN_TRY(staged_d2h(data, p->host_buf.ptr, n * eb, nullptr));
     return GSN_OK;
 }
 

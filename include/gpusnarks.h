@@ -121,10 +121,19 @@ int gsn_fft_host(int field, void *data, size_t n, const void *omega, int inverse
 /* --- domain helpers: the elementwise steps around a transform ----------
  * None has a counterpart in the reference (its FFT, cuda/fft_kernel.cu:117-147,
  * is the only operation on the path).  Device buffers of n elements (d_out may
- * equal the input), asynchronous on `stream`, canonical outputs.  Elements and
- * constants are in Montgomery form except where a name says otherwise. */
-/* d_out[i] = d_a[i] * d_b[i] (evaluation-form product). */
+ * equal the input), asynchronous on `stream`, canonical inputs and outputs.
+ * Elements and constants are in Montgomery form except where a name says
+ * otherwise. */
+/* d_out[i] = d_a[i] * d_b[i] (evaluation-form product; the reference's
+ * Scalar operator*, cuda/device_field.h:115-122, over a vector). */
 int gsn_pointwise_mul(int field, void *d_out, const void *d_a, const void *d_b, size_t n, void *stream);
+/* Vector forms of the field operators, which the reference has only per
+ * element: Scalar operator+ / operator- (cuda/device_field.h:124-140) and
+ * operator^ (cuda/device_field.h:151-158, a uint32 exponent there; 64 bits
+ * here).  Inputs canonical; x^0 = 1 for every x, 0 included. */
+int gsn_vec_add(int field, void *d_out, const void *d_a, const void *d_b, size_t n, void *stream);
+int gsn_vec_sub(int field, void *d_out, const void *d_a, const void *d_b, size_t n, void *stream);
+int gsn_vec_pow(int field, void *d_out, const void *d_in, size_t n, uint64_t exponent, void *stream);
 /* d_out[i] = d_in[i] * s, s one host element (e.g. n^-1). */
 int gsn_scale(int field, void *d_out, const void *d_in, size_t n, const void *s, void *stream);
 /* d_out[i] = d_in[i] * g^i (inverse != 0: g^-i), n a power of two, g a nonzero

@@ -1,6 +1,7 @@
 """GPU parity of the domain helpers (pointwise product, scaling, coset shift,
-Montgomery conversions) against the CPU oracle's field operations, bit-exact,
-and the coset transform's round trip at full size."""
+Montgomery conversions) and of the vector field operators (add, sub, pow)
+against the CPU oracle's field operations, bit-exact, and the coset
+transform's round trip and operator identities at full size."""
 import sys
 
 import numpy as np
@@ -52,6 +53,57 @@ def test_pointwise_and_scale(orc, field):
     g.pointwise_mul(field, da.ptr, da.ptr, db.ptr, n)
     g.synchronize()
     assert np.array_equal(da.download(a.shape), orc.mul(field, a, b))
+
+
+@pytest.mark.parametrize("field", FIELDS)
+def test_vector_operators(orc, field):
+    """gsn_vec_add / gsn_vec_sub / gsn_pointwise_mul / gsn_vec_pow against the
+    oracle's operators, element by element, edge values included."""
+    n = 1000
+    a, b = residues(field, n, 0x1A1 + field), residues(field, n, 0x2B2 + field)
+    b[4:8] = a[0:4]  # equal operands: a - a = 0, a + a
+    say("vector operators field", field)
+    da, db, dout = g.DeviceArray.from_numpy(a), g.DeviceArray.from_numpy(b), g.DeviceArray(a.nbytes)
+    g.vec_add(field, dout.ptr, da.ptr, db.ptr, n)
+    g.synchronize()
+    assert np.array_equal(dout.download(a.shape), orc.add(field, a, b))
+    g.vec_sub(field, dout.ptr, da.ptr, db.ptr, n)
+    g.synchronize()
+    assert np.array_equal(dout.download(a.shape), orc.sub(field, a, b))
+    g.vec_sub(field, dout.ptr, db.ptr, da.ptr, n)
+    g.synchronize()
+    assert np.array_equal(dout.download(a.shape), orc.sub(field, b, a))
+    g.pointwise_mul(field, dout.ptr, da.ptr, da.ptr, n)  # squares
+    g.synchronize()
+    assert np.array_equal(dout.download(a.shape), orc.mul(field, a, a))
+    for e in [0, 1, 2, 3, 0xFFFFFFFF, 0x8000000000000001, 0xFFFFFFFFFFFFFFFF, 0x5EED5EED1234]:
+        g.vec_pow(field, dout.ptr, da.ptr, n, e)
+        g.synchronize()
+        assert np.array_equal(dout.download(a.shape), orc.pow(field, a, [e] * n)), hex(e)
+
+
+def test_vector_identities_full_size():
+    """At 2^20 elements, without the oracle: (a + b) - b = a and, for the
+    32-bit field, Fermat's x^(p-1) = 1 for x != 0 (0^(p-1) = 0)."""
+    field = g.FIELD_FP32
+    n = 1 << 20
+    p = g.MODULI[field]
+    a = synth.uniform_residues(0x3C3, n, p, 1)
+    b = synth.uniform_residues(0x4D4, n, p, 1)
+    say("vector identities n", n)
+    da, db, ds = g.DeviceArray.from_numpy(a), g.DeviceArray.from_numpy(b), g.DeviceArray(a.nbytes)
+    g.vec_add(field, ds.ptr, da.ptr, db.ptr, n)
+    g.vec_sub(field, ds.ptr, ds.ptr, db.ptr, n)
+    g.vec_pow(field, db.ptr, da.ptr, n, p - 1)
+    g.synchronize()
+    assert np.array_equal(ds.download(a.shape), a)
+    one = int(orc_params_one(field))
+    expect = np.where(a[:, 0] == 0, 0, one).astype(np.uint32)
+    assert np.array_equal(db.download(a.shape)[:, 0], expect)
+
+
+def orc_params_one(field):
+    return (1 << (32 * g.NLIMBS[field])) % g.MODULI[field]
 
 
 @pytest.mark.parametrize("field", FIELDS)
@@ -120,3 +172,7 @@ def test_domain_argument_errors():
         g.pointwise_mul(7, d.ptr, d.ptr, d.ptr, 16)
     with pytest.raises(g.GsnError):
         g.scale(g.FIELD_FP32, d.ptr, d.ptr, 0, np.array([1], np.uint32))
+    with pytest.raises(g.GsnError):
+        g.vec_add(g.FIELD_FP32, d.ptr, d.ptr, None, 16)
+    with pytest.raises(g.GsnError):
+        g.vec_pow(5, d.ptr, d.ptr, 16, 3)

@@ -87,6 +87,30 @@ def test_roundtrip_and_linearity_full_size(field, log_n):
         assert (g.best_fft(d, None, field) == one).all()
 
 
+@pytest.mark.parametrize("field,log_n", [(g.FIELD_MNT6753_FR, 17), (g.FIELD_FP32, 23), (g.FIELD_FP32, 21)])
+def test_host_entry_staging_matches_device_path(field, log_n):
+    """The host entry point moves buffers above one 8 MiB staging chunk
+    through pinned double buffering (12 MiB: a ragged last chunk; 32 MiB: four
+    chunks; 8 MiB: the plain copy): its result equals the plan on device
+    buffers, and the caller's other memory is untouched."""
+    say("host staging", NAMES[field], log_n)
+    x = inputs(field, log_n, 0x57A6 + log_n)
+    plan = g.Plan(field, log_n)
+    d_in, d_out = g.DeviceArray.from_numpy(x), g.DeviceArray(x.nbytes)
+    plan.execute(d_in.ptr, d_out.ptr, 0)
+    g.synchronize()
+    want = d_out.download(x.shape)
+    L = g.NLIMBS[field]
+    guard = np.full((x.shape[0] + 64, L), 0xDEADBEEF, np.uint32)
+    guard[32:-32] = x
+    view = guard[32:-32]
+    assert view.flags["C_CONTIGUOUS"]
+    import ctypes
+    g.check(g.lib().gsn_fft_host(field, view.ctypes.data_as(ctypes.c_void_p), x.shape[0], None, 0))
+    assert np.array_equal(view, want)
+    assert (guard[:32] == 0xDEADBEEF).all() and (guard[-32:] == 0xDEADBEEF).all()
+
+
 def test_plan_api_device_buffers(orc):
     field, log_n = g.FIELD_MNT6753_FR, 12
     x = inputs(field, log_n, 77)
