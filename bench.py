@@ -210,6 +210,27 @@ def bench_configs(threads, skip_cpu):
             r["frac_of_mad_peak"] = bfly(log_n) * MADS_PER_BFLY_753 / (ms * 1e-3) / MAD_PEAK
         rows.append(r)
         log("config %s 2^%d%s: %.4f ms" % (name, log_n, " inverse" if inv else "", ms))
+    # MNT4-753 Fr at 2^16, one size past its 2-adicity, over the extended
+    # domain H u g*H (two 2^15 transforms and one elementwise kernel)
+    for inv in (False, True):
+        x = inputs(g.FIELD_MNT4753_FR, 16, 0x5EED0010)
+        plan = g.Plan.extended(g.FIELD_MNT4753_FR, 16, inverse=inv)
+        d_in, d_out = g.DeviceArray.from_numpy(x), g.DeviceArray(x.nbytes)
+        s = g.Stream()
+        for _ in range(2):
+            plan.execute(d_in.ptr, d_out.ptr, s.handle)
+        e0, e1 = g.Event(), g.Event()
+        e0.record(s.handle)
+        for _ in range(20):
+            plan.execute(d_in.ptr, d_out.ptr, s.handle)
+        e1.record(s.handle)
+        e1.synchronize()
+        ms = g.Event.elapsed_ms(e0, e1) / 20
+        rows.append({"field": "mnt4", "log2_n": 16, "inverse": inv, "domain": "extended radix-2 (H u gH)",
+                     "ms": ms, "passes": plan.num_passes, "butterflies_per_s": bfly(16) / (ms * 1e-3),
+                     "elements_per_s": (1 << 16) / (ms * 1e-3),
+                     "frac_of_mad_peak": bfly(16) * MADS_PER_BFLY_753 / (ms * 1e-3) / MAD_PEAK})
+        log("config mnt4 2^16 extended%s: %.4f ms" % (" inverse" if inv else "", ms))
     out = {"plain": rows}
     if not skip_cpu:
         out["cpu_fp32"] = [cpu_baseline(l, threads, g.FIELD_FP32, serial=(l == 20)) for l in (20, 22)]

@@ -78,6 +78,8 @@ def lib():
     L.gsn_fft_host.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_int]
     L.gsn_plan_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_uint,
                                   ctypes.c_void_p, ctypes.c_int]
+    L.gsn_plan_create_ext.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_uint,
+                                      ctypes.c_void_p, ctypes.c_int]
     L.gsn_plan_destroy.argtypes = [ctypes.c_void_p]
     L.gsn_plan_execute.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
     L.gsn_plan_device_bytes.restype = ctypes.c_size_t
@@ -245,6 +247,18 @@ class Plan:
         h = ctypes.c_void_p()
         check(lib().gsn_plan_create(ctypes.byref(h), field, log_n, wp, int(self.inverse)))
         self.handle = h
+
+    @classmethod
+    def extended(cls, field, log_n, shift=None, inverse=False):
+        """Plan over the extended domain H u shift*H (|H| = 2^(log_n - 1)),
+        which reaches log_n = 2-adicity + 1 (MNT4-753 Fr at 2^16)."""
+        self = cls.__new__(cls)
+        self.field, self.log_n, self.inverse = field, log_n, bool(inverse)
+        self._omega, sp = _omega_ptr(shift, field)
+        h = ctypes.c_void_p()
+        check(lib().gsn_plan_create_ext(ctypes.byref(h), field, log_n, sp, int(self.inverse)))
+        self.handle = h
+        return self
 
     @property
     def n(self):

@@ -70,6 +70,9 @@ struct F32 {
         return bw ? r + P : r;
     }
     __device__ __forceinline__ static uint32_t canon(uint32_t a) { return a; }
+    // the canonical sum and difference (add and sub already reduce)
+    __device__ __forceinline__ static uint32_t add_mod(uint32_t a, uint32_t b) { return add(a, b); }
+    __device__ __forceinline__ static uint32_t sub_mod(uint32_t a, uint32_t b) { return sub(a, b); }
 
     __device__ __forceinline__ static elem_t load(const elem_t *base, size_t i) { return base[i]; }
     __device__ __forceinline__ static void store(elem_t *base, size_t i, const elem_t &v) { base[i] = v; }

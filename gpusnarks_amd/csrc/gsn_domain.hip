@@ -25,6 +25,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <tuple>
 #include <vector>
 
@@ -122,6 +123,42 @@ __global__ __launch_bounds__(256) void ew_pow_kernel(typename O::elem *out, cons
             r = O::mul(r, x);
     }
     O::store(out, i, r);
+}
+
+// Extended radix-2 domain H u g*H, |H| = m (libfqfft's
+// extended_radix2_domain construction; the reference has no counterpart).
+// Forward, before the two size-m transforms: out[i] = a_i + a_(i+m) and
+// out[i+m] = g^i (a_i + G a_(i+m)), G = g^m.
+template <class F>
+__global__ __launch_bounds__(256) void ext_fwd_kernel(typename F::elem_t *out, const typename F::elem_t *in,
+                                                     const typename F::tw_t *lo, const typename F::tw_t *hi,
+                                                     uint32_t h, typename F::tw_t G, size_t m)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m)
+        return;
+    const typename F::elem_t u = F::load(in, i), w = F::load(in, i + m);
+    typename F::elem_t t = F::add_mod(u, F::canon(F::mul(w, G)));
+    t = F::canon(F::mul(F::canon(F::mul(t, F::load_tw(lo, i & ((1u << h) - 1u)))), F::load_tw(hi, i >> h)));
+    F::store(out, i, F::add_mod(u, w));
+    F::store(out, i + m, t);
+}
+// Inverse, after the two size-m inverse transforms (u_i, then g^i v_i):
+// c_(i+m) = (u_i - v_i) K with K = 1 / (1 - G), c_i = u_i - c_(i+m).
+template <class F>
+__global__ __launch_bounds__(256) void ext_inv_kernel(typename F::elem_t *a, const typename F::tw_t *lo,
+                                                     const typename F::tw_t *hi, uint32_t h, typename F::tw_t K,
+                                                     size_t m)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m)
+        return;
+    const typename F::elem_t u = F::load(a, i);
+    typename F::elem_t v = F::load(a, i + m);
+    v = F::canon(F::mul(F::canon(F::mul(v, F::load_tw(lo, i & ((1u << h) - 1u)))), F::load_tw(hi, i >> h)));
+    const typename F::elem_t c_hi = F::canon(F::mul(F::sub_mod(u, v), K));
+    F::store(a, i, F::sub_mod(u, c_hi));
+    F::store(a, i + m, c_hi);
 }
 
 // ------------------------------------------------------------ host side
@@ -255,6 +292,87 @@ int binop(int field, void *d_out, const void *d_a, const void *d_b, size_t n, vo
     return hip_status(hipGetLastError(), "elementwise launch");
 }
 
+// the field's smallest quadratic non-residue, the default shift of an
+// extended domain (g^m != 1 for every m dividing the 2-power part of p - 1)
+template <int FID>
+typename FieldInfo<FID>::Host default_shift();
+template <>
+fields::Fp32 default_shift<GSN_FIELD_FP32>() { return fields::Fp32(Fp32Params::GENERATOR); }
+template <>
+fields::Scalar default_shift<GSN_FIELD_MNT4753_FR>() { return fields::Scalar(Mnt4753FrParams::QNR); }
+template <>
+fields::ScalarMNT6 default_shift<GSN_FIELD_MNT6753_FR>() { return fields::ScalarMNT6(Mnt6753FrParams::QNR); }
+
+// A transform over the extended domain H u g*H of n = 2m points: one
+// elementwise kernel and two size-m plans (forward: combine, then
+// transform both halves; inverse: transform both halves, then solve).
+template <int FID>
+struct ExtPlan : PlanBase {
+    using H = typename FieldInfo<FID>::Host;
+    using F = typename FieldInfo<FID>::Dev;
+    PlanBase *half = nullptr;
+    CosetTables *tab = nullptr;  // g^i (forward) or g^-i (inverse), i < m
+    typename F::tw_t c;          // G (forward) or 1 / (1 - G) (inverse)
+    ~ExtPlan() override { delete half; }
+
+    int init(unsigned L, const H &g, int inv)
+    {
+        field = FID;
+        log_n = L;
+        inverse = inv;
+        const unsigned lm = L - 1;
+        const H G = g.pow((uint64_t)1 << lm);
+        if (G == H::one())
+            return set_error(GSN_ERR_INVALID_ARGUMENT, "shift^(n/2) = 1: g*H would meet H");
+        GSN_TRY(create_plan(&half, FID, lm, nullptr, inv));
+        GSN_TRY(coset_tables<FID>(&tab, g, lm, inv));
+        c = inv ? record<FID>((H::one() - G).mul_inv()) : record<FID>(G);
+        num_passes = 2 * half->num_passes + 1;
+        device_bytes = half->device_bytes;
+        return GSN_OK;
+    }
+
+    int execute(const void *in, void *out, hipStream_t st) override
+    {
+        using E = typename F::elem_t;
+        const size_t m = (size_t)1 << (log_n - 1);
+        const size_t hb = m * sizeof(E);
+        const dim3 grid(blocks_for(m)), block(256);
+        const auto *lo = (const typename F::tw_t *)tab->lo.ptr, *hi = (const typename F::tw_t *)tab->hi.ptr;
+        if (!inverse) {
+            GSN_TRY(take_injected_fault());
+            hipLaunchKernelGGL(ext_fwd_kernel<F>, grid, block, 0, st, (E *)out, (const E *)in, lo, hi, tab->h, c, m);
+            GSN_TRY(hip_status(hipGetLastError(), "extended-domain launch"));
+            GSN_TRY(half->execute(out, out, st));
+            return half->execute((char *)out + hb, (char *)out + hb, st);
+        }
+        GSN_TRY(half->execute(in, out, st));
+        GSN_TRY(half->execute((const char *)in + hb, (char *)out + hb, st));
+        GSN_TRY(take_injected_fault());
+        hipLaunchKernelGGL(ext_inv_kernel<F>, grid, block, 0, st, (E *)out, lo, hi, tab->h, c, m);
+        return hip_status(hipGetLastError(), "extended-domain launch");
+    }
+};
+
+template <int FID>
+int create_ext_plan(PlanBase **out, unsigned log_n, const void *shift, int inverse)
+{
+    using H = typename FieldInfo<FID>::Host;
+    if (log_n < 1 || log_n - 1 > FieldInfo<FID>::TWO_ADICITY)
+        return set_error(GSN_ERR_SIZE_EXCEEDS_TWO_ADICITY, "extended domain of 2^" + std::to_string(log_n) +
+                                                       " points needs 1 <= log_n <= 2-adicity + 1");
+    H g = default_shift<FID>();
+    if (shift) {
+        g = H::from_raw((const uint32_t *)shift);
+        if (fields::detail::geq<H::LIMBS>(g.im_rep, H::mod()) || g == H::zero())
+            return set_error(GSN_ERR_INVALID_ARGUMENT, "the shift must be a nonzero canonical element");
+    }
+    std::unique_ptr<ExtPlan<FID>> p(new ExtPlan<FID>());
+    GSN_TRY(p->init(log_n, g, inverse));
+    *out = p.release();
+    return GSN_OK;
+}
+
 }  // namespace
 
 }  // namespace gsn
@@ -300,6 +418,23 @@ int gsn_vec_pow(int field, void *d_out, const void *d_in, size_t n, uint64_t exp
         break;
     }
     return hip_status(hipGetLastError(), "pow launch");
+}
+
+int gsn_plan_create_ext(gsn_plan **plan, int field, unsigned log_n, const void *shift, int inverse)
+{
+    if (!plan)
+        return set_error(GSN_ERR_INVALID_ARGUMENT, "plan is NULL");
+    *plan = nullptr;
+    GSN_TRY(ensure_device());
+    PlanBase *impl = nullptr;
+    switch (field) {
+    case GSN_FIELD_FP32: GSN_TRY(create_ext_plan<GSN_FIELD_FP32>(&impl, log_n, shift, inverse)); break;
+    case GSN_FIELD_MNT4753_FR: GSN_TRY(create_ext_plan<GSN_FIELD_MNT4753_FR>(&impl, log_n, shift, inverse)); break;
+    case GSN_FIELD_MNT6753_FR: GSN_TRY(create_ext_plan<GSN_FIELD_MNT6753_FR>(&impl, log_n, shift, inverse)); break;
+    default: return set_error(GSN_ERR_INVALID_ARGUMENT, "unknown field id");
+    }
+    *plan = new gsn_plan{impl};
+    return GSN_OK;
 }
 
 int gsn_scale(int field, void *d_out, const void *d_in, size_t n, const void *s, void *stream)

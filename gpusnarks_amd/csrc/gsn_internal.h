@@ -68,3 +68,8 @@ void clear_plan_cache();
 int cached_plan(PlanBase **out, int field, unsigned log_n, const void *omega, int inverse);
 
 }  // namespace gsn
+
+// the C ABI's opaque plan handle
+struct gsn_plan {
+    gsn::PlanBase *impl;
+};

@@ -484,10 +484,6 @@ int cached_plan(PlanBase **out, int field, unsigned log_n, const void *omega, in
 
 using namespace gsn;
 
-struct gsn_plan {
-    PlanBase *impl;
-};
-
 extern "C" {
 
 const char *gsn_strerror(int status)

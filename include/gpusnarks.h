@@ -148,6 +148,16 @@ int gsn_from_montgomery(int field, void *d_out, const void *d_in, size_t n, void
 typedef struct gsn_plan gsn_plan;
 int gsn_plan_create(gsn_plan **plan, int field, unsigned log_n, const void *omega, int inverse);
 int gsn_plan_destroy(gsn_plan *plan);
+/* A plan over the extended radix-2 domain H u shift*H of n = 2^log_n points
+ * (|H| = n/2, H generated by the standard primitive (n/2)-th root): forward
+ * evaluates a polynomial of n coefficients on H (outputs 0..n/2-1, natural
+ * order) and then on shift*H (outputs n/2..n-1); inverse interpolates back.
+ * It reaches one size past the field's 2-adicity, e.g. MNT4-753 Fr at 2^16.
+ * shift NULL: the field's smallest quadratic non-residue; shift^(n/2) must
+ * not be 1.  No counterpart in the reference, whose FFT is power-of-two
+ * radix-2 only (cuda/fft_kernel.cu:117-147); the construction is libfqfft's
+ * extended_radix2_domain.  Executes with gsn_plan_execute. */
+int gsn_plan_create_ext(gsn_plan **plan, int field, unsigned log_n, const void *shift, int inverse);
 /* Transforms d_in into d_out (device pointers; d_out may equal d_in) on
  * `stream`.  Asynchronous; the plan's workspace is reused, so calls on one
  * plan must not overlap in time. */

@@ -431,3 +431,52 @@ void or_inverse_fft(const or_field *f, uint32_t *a, size_t n, const uint32_t *om
     for (size_t i = 0; i < n; ++i)
         or_mul(f, EL(a, i, s), EL(a, i, s), ninv);
 }
+
+/* Extended radix-2 domain H u shift*H, |H| = m = n/2, the construction of
+ * libfqfft's extended_radix2_domain (not vendored with the reference; its
+ * published algorithm): it lets a field of 2-adicity s evaluate on 2^(s+1)
+ * points.  omega_half is a primitive m-th root, shift^m != 1.
+ * Forward: with G = shift^m, A(w^i) is the size-m DFT of a_j + a_(j+m) and
+ * A(shift w^i) that of shift^j (a_j + G a_(j+m)); outputs in that order.
+ * Inverse: the two size-m inverse DFTs give u_j = c_j + c_(j+m) and
+ * shift^j (c_j + G c_(j+m)); then c_(j+m) = (u - v) / (1 - G) with
+ * v = shift^-j * (second), and c_j = u - c_(j+m). */
+void or_extended_radix2_fft(const or_field *f, uint32_t *a, size_t n, const uint32_t *omega_half,
+                            const uint32_t *shift, int inverse)
+{
+    const int s = f->nlimbs;
+    const size_t m = n / 2;
+    uint32_t G[OR_MAX_LIMBS], t[OR_MAX_LIMBS], u[OR_MAX_LIMBS], v[OR_MAX_LIMBS], sh_i[OR_MAX_LIMBS];
+    or_pow(f, G, shift, (uint64_t)m);
+    if (!inverse) {
+        memcpy(sh_i, f->one, sizeof(uint32_t) * s);
+        for (size_t i = 0; i < m; ++i) {
+            uint32_t *lo = EL(a, i, s), *hi = EL(a, i + m, s);
+            or_mul(f, t, G, hi);
+            or_add(f, t, t, lo);
+            or_mul(f, t, t, sh_i);
+            or_add(f, lo, lo, hi);
+            memcpy(hi, t, sizeof(uint32_t) * s);
+            or_mul(f, sh_i, sh_i, shift);
+        }
+        or_serial_radix2_fft(f, a, m, omega_half, f->one, 0);
+        or_serial_radix2_fft(f, EL(a, m, s), m, omega_half, f->one, 0);
+        return;
+    }
+    or_inverse_fft(f, a, m, omega_half);
+    or_inverse_fft(f, EL(a, m, s), m, omega_half);
+    uint32_t shinv[OR_MAX_LIMBS], k[OR_MAX_LIMBS];
+    or_inverse(f, shinv, shift);
+    or_sub(f, k, f->one, G);
+    or_inverse(f, k, k); /* 1 / (1 - G) */
+    memcpy(sh_i, f->one, sizeof(uint32_t) * s);
+    for (size_t i = 0; i < m; ++i) {
+        uint32_t *lo = EL(a, i, s), *hi = EL(a, i + m, s);
+        memcpy(u, lo, sizeof(uint32_t) * s);
+        or_mul(f, v, hi, sh_i);
+        or_sub(f, t, u, v);
+        or_mul(f, hi, t, k);
+        or_sub(f, lo, u, hi);
+        or_mul(f, sh_i, sh_i, shinv);
+    }
+}

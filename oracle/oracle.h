@@ -82,6 +82,10 @@ void or_parallel_phase1_sample(const or_field *f, const uint32_t *a, size_t n,
 
 /* Inverse transform: forward transform with omega^-1, then scale by n^-1. */
 void or_inverse_fft(const or_field *f, uint32_t *a, size_t n, const uint32_t *omega);
+/* Extended radix-2 domain H u shift*H (|H| = n/2, omega_half generates H):
+ * forward evaluates on H then shift*H; inverse interpolates back. */
+void or_extended_radix2_fft(const or_field *f, uint32_t *a, size_t n, const uint32_t *omega_half,
+                            const uint32_t *shift, int inverse);
 
 /* omega^-1 (Fermat inversion). */
 void or_inverse(const or_field *f, uint32_t *r, const uint32_t *a);

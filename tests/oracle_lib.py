@@ -48,6 +48,8 @@ class Oracle:
         L.or_parallel_phase1_sample.argtypes = [ctypes.POINTER(OrField), _u32p, ctypes.c_size_t, _u32p,
                                                 ctypes.c_uint, ctypes.c_size_t, ctypes.c_size_t, _u32p, ctypes.c_int]
         L.or_inverse_fft.argtypes = [ctypes.POINTER(OrField), _u32p, ctypes.c_size_t, _u32p]
+        L.or_extended_radix2_fft.argtypes = [ctypes.POINTER(OrField), _u32p, ctypes.c_size_t, _u32p, _u32p,
+                                             ctypes.c_int]
         self.lib = L
 
     def field(self, fid):
@@ -103,6 +105,13 @@ class Oracle:
         self.lib.or_parallel_radix2_fft_inner(self.field(fid), _p(a), a.shape[0],
                                               _p(np.ascontiguousarray(omega, np.uint32)), log_cpus,
                                               _p(one), int(literal), threads)
+        return a
+
+    def extended_fft(self, fid, x, omega_half, shift, inverse=False):
+        a = np.ascontiguousarray(x, np.uint32).copy()
+        self.lib.or_extended_radix2_fft(self.field(fid), _p(a), a.shape[0],
+                                        _p(np.ascontiguousarray(omega_half, np.uint32)),
+                                        _p(np.ascontiguousarray(shift, np.uint32)), int(inverse))
         return a
 
     def inverse_fft(self, fid, x, omega):

@@ -176,3 +176,47 @@ def test_domain_argument_errors():
         g.vec_add(g.FIELD_FP32, d.ptr, d.ptr, None, 16)
     with pytest.raises(g.GsnError):
         g.vec_pow(5, d.ptr, d.ptr, 16, 3)
+
+
+def mont(field, v):
+    L = g.NLIMBS[field]
+    return synth.from_int(v * (1 << (32 * L)) % g.MODULI[field], L)
+
+
+QNR = {g.FIELD_FP32: 5, g.FIELD_MNT4753_FR: 13, g.FIELD_MNT6753_FR: 11}
+
+
+@pytest.mark.parametrize("field,log_n,custom_shift", [
+    (g.FIELD_FP32, 10, False), (g.FIELD_MNT4753_FR, 9, False), (g.FIELD_MNT6753_FR, 9, True),
+    (g.FIELD_FP32, 21, True), (g.FIELD_MNT4753_FR, 16, False)])
+def test_extended_domain_matches_oracle(orc, field, log_n, custom_shift):
+    """Plan.extended (H u g*H) against the oracle's extended-domain FFT,
+    forward and inverse, bit-exact; MNT4-753 Fr at 2^16 is one size past its
+    2-adicity."""
+    n = 1 << log_n
+    x = synth.uniform_residues(0xEE0 + log_n, n, g.MODULI[field], g.NLIMBS[field])
+    shift = residues(field, 8, 0x5F + log_n)[6] if custom_shift else mont(field, QNR[field])
+    say("extended domain field", field, "log_n", log_n, "custom shift", custom_shift)
+    fwd = g.Plan.extended(field, log_n, shift if custom_shift else None)
+    inv = g.Plan.extended(field, log_n, shift if custom_shift else None, inverse=True)
+    d, e = g.DeviceArray.from_numpy(x), g.DeviceArray(x.nbytes)
+    fwd.execute(d.ptr, e.ptr)
+    g.synchronize()
+    got = e.download(x.shape)
+    omega_half = g.root_of_unity(field, log_n - 1)
+    if n <= (1 << 16):
+        want = orc.extended_fft(field, x, omega_half, shift)
+        assert np.array_equal(got, want)
+    inv.execute(e.ptr, e.ptr)  # in place
+    g.synchronize()
+    assert np.array_equal(e.download(x.shape), x)
+
+
+def test_extended_domain_errors():
+    one = mont(g.FIELD_MNT6753_FR, 1)
+    with pytest.raises(g.GsnError):
+        g.Plan.extended(g.FIELD_MNT4753_FR, 17)  # 2-adicity 15: at most 2^16 points
+    with pytest.raises(g.GsnError):
+        g.Plan.extended(g.FIELD_MNT6753_FR, 8, one)  # shift in H: g*H = H
+    with pytest.raises(g.GsnError):
+        g.Plan.extended(g.FIELD_FP32, 0)

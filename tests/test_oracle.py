@@ -221,3 +221,50 @@ def test_closed_forms(orc, fid, n):
     assert np.array_equal(lhs, rhs)
     # inverse(forward(x)) == x
     assert np.array_equal(orc.inverse_fft(fid, orc.serial_fft(fid, x, om), om), x)
+
+
+def _eval_poly(coeffs, x, p):
+    acc = 0
+    for c in reversed(coeffs):
+        acc = (acc * x + c) % p
+    return acc
+
+
+@pytest.mark.parametrize("fid,m", [(0, 8), (0, 32), (1, 16), (2, 16)])
+def test_extended_domain_matches_evaluation(orc, fid, m):
+    """The extended radix-2 domain H u g*H against direct evaluation of the
+    polynomial at the 2m points (w^k, then g w^k, g the field's QNR), and the
+    inverse against the coefficients."""
+    L = NLIMBS[fid]
+    p = synth.to_int(orc.params(fid)["mod"])
+    R = 1 << (32 * L)
+    qnr = {0: 5, 1: 13, 2: 11}[fid]
+    w = pow(qnr, (p - 1) // m, p)
+    mont = lambda v: synth.from_int(v * R % p, L)
+    val = lambda a: synth.to_int(a) * pow(R, -1, p) % p
+    c = synth.uniform_residues(0xE1 + m, 2 * m, p, L)
+    cv = [val(r) for r in c]
+    out = orc.extended_fft(fid, c, mont(w), mont(qnr))
+    points = [pow(w, k, p) for k in range(m)] + [qnr * pow(w, k, p) % p for k in range(m)]
+    assert [val(r) for r in out] == [_eval_poly(cv, x, p) for x in points]
+    assert np.array_equal(orc.extended_fft(fid, out, mont(w), mont(qnr), inverse=True), c)
+
+
+def test_extended_domain_mnt4_full_size(orc):
+    """MNT4-753 Fr (2-adicity 15) on 2^16 points: spot evaluations of the
+    2^16-coefficient polynomial, and the round trip."""
+    fid, m = 1, 1 << 15
+    L = NLIMBS[fid]
+    p = synth.to_int(orc.params(fid)["mod"])
+    R = 1 << (32 * L)
+    w = pow(13, (p - 1) // m, p)
+    assert pow(w, m // 2, p) == p - 1
+    mont = lambda v: synth.from_int(v * R % p, L)
+    rinv = pow(R, -1, p)
+    c = synth.uniform_residues(0xE4, 2 * m, p, L)
+    out = orc.extended_fft(fid, c, mont(w), mont(13))
+    cv = [synth.to_int(r) * rinv % p for r in c]
+    for k in (0, 1, 12345, m - 1):
+        assert synth.to_int(out[k]) * rinv % p == _eval_poly(cv, pow(w, k, p), p)
+        assert synth.to_int(out[m + k]) * rinv % p == _eval_poly(cv, 13 * pow(w, k, p) % p, p)
+    assert np.array_equal(orc.extended_fft(fid, out, mont(w), mont(13), inverse=True), c)
