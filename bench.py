@@ -18,7 +18,8 @@ HBM), one transform per step.  The same line carries:
   fp32          the 32-bit field at n = 2^26 against HBM bandwidth;
   configs       every configured size as a plain transform;
   per_transform median/min/max of individually timed transforms;
-  end_to_end    the host-buffer entry point (best_fft) including its copies.
+  end_to_end    the host-buffer entry point (best_fft) including its copies;
+  inner_product the reference's multiexp over field elements at 2^18.
 
     python bench.py [--gpus 1] [--steps 50] [--warmup 5]
 """
@@ -124,6 +125,27 @@ def end_to_end(field, log_n, runs=5):
     return {"runs": runs, "median_ms": ms, "bytes_each_way": int(x.nbytes),
             "butterflies_per_s": bfly(log_n) / (ms * 1e-3),
             "note": "pageable host buffer in, transform, host buffer out (PCIe-inclusive); not the headline"}
+
+
+def bench_inner_product(log_n=18, reps=20):
+    """The reference's multiexp over Scalar x Scalar (test/main.cpp:89-132,
+    n = 2^18, MNT4-753 Fr) on device buffers: sum_i a_i b_i."""
+    import gpusnarks_amd as g
+    field = g.FIELD_MNT4753_FR
+    a = inputs(field, log_n, 0x5EED0020)
+    b = inputs(field, log_n, 0x5EED0021)
+    da, db, dout = g.DeviceArray.from_numpy(a), g.DeviceArray.from_numpy(b), g.DeviceArray(96)
+    s = g.Stream()
+    g.inner_product(field, dout.ptr, da.ptr, db.ptr, a.shape[0], s.handle)
+    e0, e1 = g.Event(), g.Event()
+    e0.record(s.handle)
+    for _ in range(reps):
+        g.inner_product(field, dout.ptr, da.ptr, db.ptr, a.shape[0], s.handle)
+    e1.record(s.handle)
+    e1.synchronize()
+    ms = g.Event.elapsed_ms(e0, e1) / reps
+    return {"field": "MNT4-753 Fr", "n": a.shape[0], "ms": ms, "products_per_s": a.shape[0] / (ms * 1e-3),
+            "note": "general 753-bit Montgomery products (1431 multiply-adds each) plus a two-level sum"}
 
 
 def cpu_info():
@@ -499,6 +521,8 @@ def main():
             "actual_pass_bytes_GBps": 8 * (1 << 26) * p32 / (ms32 * 1e-3) / 1e9,
             "per_transform": step_times(g.FIELD_FP32, 26),
         }
+        log("inner product")
+        line["inner_product"] = bench_inner_product()
         log("end to end (host buffers)")
         line["end_to_end"] = {"mnt6_n18": end_to_end(g.FIELD_MNT6753_FR, log_n),
                               "fp32_n26": end_to_end(g.FIELD_FP32, 26)}

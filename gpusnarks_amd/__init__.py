@@ -104,6 +104,7 @@ def lib():
     L.gsn_debug_copy_u32.argtypes = [vp, vp, sz, vp]
     L.gsn_debug_clock_ghz.argtypes = [ctypes.POINTER(ctypes.c_double)]
     L.gsn_pointwise_mul.argtypes = [ctypes.c_int, vp, vp, vp, sz, vp]
+    L.gsn_inner_product.argtypes = [ctypes.c_int, vp, vp, vp, sz, vp]
     L.gsn_vec_add.argtypes = [ctypes.c_int, vp, vp, vp, sz, vp]
     L.gsn_vec_sub.argtypes = [ctypes.c_int, vp, vp, vp, sz, vp]
     L.gsn_vec_pow.argtypes = [ctypes.c_int, vp, vp, sz, ctypes.c_uint64, vp]
@@ -182,6 +183,13 @@ def _elem_ptr(v, field):
 def pointwise_mul(field, d_out, d_a, d_b, n, stream=None):
     """d_out[i] = d_a[i] * d_b[i] (Montgomery forms)."""
     check(lib().gsn_pointwise_mul(field, ctypes.c_void_p(d_out), ctypes.c_void_p(d_a), ctypes.c_void_p(d_b), n,
+                                  stream))
+
+
+def inner_product(field, d_out, d_a, d_b, n, stream=None):
+    """*d_out = sum_i d_a[i] * d_b[i] on device (the reference's multiexp over
+    Scalar x Scalar, cuda/multi_exp.cu:104-135)."""
+    check(lib().gsn_inner_product(field, ctypes.c_void_p(d_out), ctypes.c_void_p(d_a), ctypes.c_void_p(d_b), n,
                                   stream))
 
 

@@ -22,6 +22,7 @@
 // per (field, g, n, direction).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -70,6 +71,7 @@ struct Ops32 {
     __device__ static elem sub(elem a, elem b) { return F32::sub(a, b); }
     __device__ static elem mul(elem a, elem b) { return F32::mul(a, F32::tw_t{b, b * Fp32Params::PINV}); }
     __device__ static elem one() { return Fp32Params::ONE; }
+    __device__ static elem zero() { return 0u; }
     __device__ static elem load(const elem *p, size_t i) { return p[i]; }
     __device__ static void store(elem *p, size_t i, elem v) { p[i] = v; }
 };
@@ -86,6 +88,14 @@ struct Ops753 {
 #pragma unroll
         for (int i = 0; i < 24; ++i)
             o.v[i] = P::ONE[i];
+        return o;
+    }
+    __device__ static elem zero()
+    {
+        elem o;
+#pragma unroll
+        for (int i = 0; i < 24; ++i)
+            o.v[i] = 0u;
         return o;
     }
     __device__ static elem load(const elem *p, size_t i) { return F::load(p, i); }
@@ -123,6 +133,31 @@ __global__ __launch_bounds__(256) void ew_pow_kernel(typename O::elem *out, cons
             r = O::mul(r, x);
     }
     O::store(out, i, r);
+}
+
+// Inner product sum_i a_i b_i (the reference's multiexp over field
+// elements, cuda/multi_exp.cu:88-100,104-135 with FieldT = FieldMul =
+// Scalar): each block sums a grid-stride slice of products, then a tree in
+// LDS; `b` null sums `a` alone (the second level, over the block sums).
+constexpr int DOT_THREADS = 256;
+template <class O>
+__global__ __launch_bounds__(DOT_THREADS) void dot_kernel(typename O::elem *partial, const typename O::elem *a,
+                                                         const typename O::elem *b, size_t n)
+{
+    __shared__ typename O::elem sm[DOT_THREADS];
+    const int t = threadIdx.x;
+    typename O::elem acc = O::zero();
+    for (size_t i = (size_t)blockIdx.x * DOT_THREADS + t; i < n; i += (size_t)gridDim.x * DOT_THREADS)
+        acc = O::add(acc, b ? O::mul(O::load(a, i), O::load(b, i)) : O::load(a, i));
+    sm[t] = acc;
+    __syncthreads();
+    for (int s = DOT_THREADS / 2; s > 0; s >>= 1) {
+        if (t < s)
+            sm[t] = O::add(sm[t], sm[t + s]);
+        __syncthreads();
+    }
+    if (t == 0)
+        O::store(partial, blockIdx.x, sm[0]);
 }
 
 // Extended radix-2 domain H u g*H, |H| = m (libfqfft's
@@ -373,6 +408,24 @@ int create_ext_plan(PlanBase **out, unsigned log_n, const void *shift, int inver
     return GSN_OK;
 }
 
+template <class O>
+int dot_t(void *d_out, const void *d_a, const void *d_b, size_t n, hipStream_t st)
+{
+    using E = typename O::elem;
+    const unsigned blocks = (unsigned)std::min<size_t>(1024, (n + DOT_THREADS - 1) / DOT_THREADS);
+    // block sums in stream-ordered scratch: concurrent calls on different
+    // streams get their own
+    void *partial = nullptr;
+    GSN_TRY(hip_status(hipMallocAsync(&partial, blocks * sizeof(E), st), "hipMallocAsync"));
+    hipLaunchKernelGGL(dot_kernel<O>, dim3(blocks), dim3(DOT_THREADS), 0, st, (E *)partial, (const E *)d_a,
+                       (const E *)d_b, n);
+    hipLaunchKernelGGL(dot_kernel<O>, dim3(1), dim3(DOT_THREADS), 0, st, (E *)d_out, (const E *)partial,
+                       (const E *)nullptr, (size_t)blocks);
+    const int rc = hip_status(hipGetLastError(), "inner product launch");
+    const int rf = hip_status(hipFreeAsync(partial, st), "hipFreeAsync");
+    return rc ? rc : rf;
+}
+
 }  // namespace
 
 }  // namespace gsn
@@ -418,6 +471,21 @@ int gsn_vec_pow(int field, void *d_out, const void *d_in, size_t n, uint64_t exp
         break;
     }
     return hip_status(hipGetLastError(), "pow launch");
+}
+
+int gsn_inner_product(int field, void *d_out, const void *d_a, const void *d_b, size_t n, void *stream)
+{
+    GSN_TRY(check_args(field, d_out, d_a, n));
+    if (!d_b)
+        return set_error(GSN_ERR_INVALID_ARGUMENT, "NULL buffer");
+    GSN_TRY(take_injected_fault());
+    GSN_TRY(ensure_device());
+    hipStream_t st = (hipStream_t)stream;
+    switch (field) {
+    case GSN_FIELD_FP32: return dot_t<Ops32>(d_out, d_a, d_b, n, st);
+    case GSN_FIELD_MNT4753_FR: return dot_t<Ops753<Mnt4753FrParams>>(d_out, d_a, d_b, n, st);
+    default: return dot_t<Ops753<Mnt6753FrParams>>(d_out, d_a, d_b, n, st);
+    }
 }
 
 int gsn_plan_create_ext(gsn_plan **plan, int field, unsigned log_n, const void *shift, int inverse)

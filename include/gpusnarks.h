@@ -134,6 +134,12 @@ int gsn_pointwise_mul(int field, void *d_out, const void *d_a, const void *d_b, 
 int gsn_vec_add(int field, void *d_out, const void *d_a, const void *d_b, size_t n, void *stream);
 int gsn_vec_sub(int field, void *d_out, const void *d_a, const void *d_b, size_t n, void *stream);
 int gsn_vec_pow(int field, void *d_out, const void *d_in, size_t n, uint64_t exponent, void *stream);
+/* *d_out = sum_i d_a[i] * d_b[i], one element written to device memory:
+ * the reference's multiexp over field elements (cuda/multi_exp.cu:104-135,
+ * instantiated for Scalar x Scalar at :139; its host version is
+ * test/multiexp.h:3-14), on device buffers and asynchronous on `stream`
+ * (block sums live in stream-ordered scratch, hipMallocAsync). */
+int gsn_inner_product(int field, void *d_out, const void *d_a, const void *d_b, size_t n, void *stream);
 /* d_out[i] = d_in[i] * s, s one host element (e.g. n^-1). */
 int gsn_scale(int field, void *d_out, const void *d_in, size_t n, const void *s, void *stream);
 /* d_out[i] = d_in[i] * g^i (inverse != 0: g^-i), n a power of two, g a nonzero

@@ -220,3 +220,34 @@ def test_extended_domain_errors():
         g.Plan.extended(g.FIELD_MNT6753_FR, 8, one)  # shift in H: g*H = H
     with pytest.raises(g.GsnError):
         g.Plan.extended(g.FIELD_FP32, 0)
+
+
+@pytest.mark.parametrize("field,n", [(g.FIELD_FP32, 3001), (g.FIELD_MNT4753_FR, 1), (g.FIELD_MNT4753_FR, 2049),
+                                     (g.FIELD_MNT6753_FR, 700)])
+def test_inner_product_matches_oracle(orc, field, n):
+    """gsn_inner_product (the reference's multiexp over Scalar x Scalar)
+    against the oracle's products folded with its additions."""
+    a, b = residues(field, max(n, 4), 0x1D0 + n)[:n], residues(field, max(n, 4), 0x2E0 + n)[:n]
+    say("inner product field", field, "n", n)
+    da, db, dout = g.DeviceArray.from_numpy(a), g.DeviceArray.from_numpy(b), g.DeviceArray(a[:1].nbytes)
+    g.inner_product(field, dout.ptr, da.ptr, db.ptr, n)
+    g.synchronize()
+    prods = orc.mul(field, a, b)
+    acc = prods[:1].copy()
+    for i in range(1, n):
+        acc = orc.add(field, acc, prods[i:i + 1])
+    assert np.array_equal(dout.download(acc.shape), acc)
+
+
+def test_inner_product_reference_harness_shape():
+    """The reference's test_multiexp (test/main.cpp:89-132): 2^18 copies of
+    Scalar(1234) against 2^18 copies of Scalar(1234); the sum is
+    2^18 * 1234^2, in Montgomery form."""
+    field, n = g.FIELD_MNT4753_FR, 1 << 18
+    p, L = g.MODULI[field], g.NLIMBS[field]
+    v = np.tile(mont(field, 1234), (n, 1))
+    say("inner product 2^18 x Scalar(1234)")
+    da, dout = g.DeviceArray.from_numpy(v), g.DeviceArray(96)
+    g.inner_product(field, dout.ptr, da.ptr, da.ptr, n)
+    g.synchronize()
+    assert np.array_equal(dout.download((1, L))[0], mont(field, n * 1234 * 1234 % p))
