@@ -4,7 +4,9 @@
 //
 //   dropin_main host   argument checks and field identities (no GPU)
 //   dropin_main gpu    best_fft / best_ifft against an O(n^2) DFT computed
-//                      with the host field type, for all three fields
+//                      with the host field type, for all three fields, and
+//                      multiexp (cuda/multi_exp.h:24-25) against the host
+//                      loop of test/multiexp.h:3-14
 #include <cstdio>
 #include <cstring>
 #include <stdexcept>
@@ -12,6 +14,7 @@
 
 #include "fft_kernel.h"
 #include "gpusnarks.h"
+#include "multi_exp.h"
 
 static int failures = 0;
 #define CHECK(c)                                                    \
@@ -66,6 +69,33 @@ void host_checks()
     // Scalar(123)): both rejected before any device work
     CHECK(throws<F>(std::vector<F>(3, F(1)), root<F>(2)));
     CHECK(throws<F>(std::vector<F>(16, F(1)), F(123)));
+    // multiexp: mismatched sizes throw (the reference asserts); empty is zero
+    std::vector<F> a(3, F(1)), b(4, F(1)), e;
+    bool threw = false;
+    try {
+        multiexp<F, F>(a, b);
+    } catch (const std::runtime_error &) {
+        threw = true;
+    }
+    CHECK(threw);
+    CHECK((multiexp<F, F>(e, e) == F::zero()));
+}
+
+// test/multiexp.h:3-14: result = result + a[i] * b[i]
+template <class F>
+F host_multi_exp(const std::vector<F> &a, const std::vector<F> &b)
+{
+    F r = F::zero();
+    for (size_t i = 0; i < a.size(); ++i)
+        r = r + a[i] * b[i];
+    return r;
+}
+
+template <class F>
+void multiexp_checks(size_t n)
+{
+    std::vector<F> a = inputs<F>(n, 0xA0A0u), b = inputs<F>(n, 0xB0B0u);
+    CHECK((multiexp<F, F>(a, b) == host_multi_exp(a, b)));
 }
 
 template <class F>
@@ -106,6 +136,12 @@ int main(int argc, char **argv)
         gpu_checks<fields::Scalar>(8);
         gpu_checks<fields::ScalarMNT6>(9);
         gpu_checks<fields::Fp32>(12);
+        multiexp_checks<fields::Scalar>(1000);
+        multiexp_checks<fields::ScalarMNT6>(333);
+        multiexp_checks<fields::Fp32>(100000);
+        // the reference harness's shape (test/main.cpp:89-132)
+        std::vector<fields::Scalar> v(1 << 18, fields::Scalar(1234));
+        CHECK((multiexp<fields::Scalar, fields::Scalar>(v, v) == host_multi_exp(v, v)));
     }
     std::printf("%s: %d failure(s)\n", gpu ? "gpu" : "host", failures);
     return failures ? 1 : 0;
