@@ -161,13 +161,39 @@ def cpu_info():
     return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0))}
 
 
+_NATIVE_ORACLE = []
+
+
+def native_oracle():
+    """The oracle compiled for this host's CPU (gcc -O3 -march=native
+    -fopenmp) into a temporary directory, as the CPU baseline asks; the
+    shipped oracle/liboracle.so (built without -march) if that fails."""
+    import subprocess
+    import tempfile
+    from tests.oracle_lib import ORACLE_SO
+    if not _NATIVE_ORACLE:
+        path, flags = ORACLE_SO, "oracle/liboracle.so (-O3 -fopenmp)"
+        try:
+            d = tempfile.mkdtemp(prefix="gsn_oracle_")
+            so = os.path.join(d, "liboracle_native.so")
+            r = subprocess.run(["gcc", "-O3", "-march=native", "-fopenmp", "-fPIC", "-shared", "-o", so,
+                                os.path.join(ROOT, "oracle", "oracle.c")], capture_output=True, timeout=120)
+            if r.returncode == 0:
+                path, flags = so, "gcc -O3 -march=native -fopenmp (built on this host)"
+        except (OSError, subprocess.SubprocessError):
+            pass
+        _NATIVE_ORACLE.append((path, flags))
+    return _NATIVE_ORACLE[0]
+
+
 def cpu_baseline(log_n, threads, field=None, serial=True):
     """Oracle (test/fft_host.h restated) on the host: the reference
     harness's parallel radix-2 FFT with log_cpus = 10, bounded sample."""
     import ctypes
     from tests.oracle_lib import Oracle
     import gpusnarks_amd as g
-    orc = Oracle()
+    so, build_flags = native_oracle()
+    orc = Oracle(so)
     field = g.FIELD_MNT6753_FR if field is None else field
     fname = {g.FIELD_FP32: "Fp32", g.FIELD_MNT4753_FR: "MNT4-753 Fr", g.FIELD_MNT6753_FR: "MNT6-753 Fr"}[field]
     n = 1 << log_n
@@ -200,6 +226,7 @@ def cpu_baseline(log_n, threads, field=None, serial=True):
         "seconds_extrapolated": total,
     }
     out.update(cpu_info())
+    out["build"] = build_flags
     if serial:
         # and the O(n log n) serial transform on one core, in full
         t0 = time.perf_counter()
@@ -255,7 +282,8 @@ def bench_configs(threads, skip_cpu):
         log("config mnt4 2^16 extended%s: %.4f ms" % (" inverse" if inv else "", ms))
     out = {"plain": rows}
     if not skip_cpu:
-        out["cpu_fp32"] = [cpu_baseline(l, threads, g.FIELD_FP32, serial=(l == 20)) for l in (20, 22)]
+        out["cpu_fp32"] = [cpu_baseline(l, threads, g.FIELD_FP32, serial=(l == 20)) for l in (16, 20, 22)]
+        out["cpu_mnt6_n16"] = cpu_baseline(16, threads, g.FIELD_MNT6753_FR, serial=False)
     return out
 
 
