@@ -81,6 +81,12 @@ def lib():
     L.gsn_plan_create_ext.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_uint,
                                       ctypes.c_void_p, ctypes.c_int]
     L.gsn_plan_destroy.argtypes = [ctypes.c_void_p]
+    L.gsn_msm_bases_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_void_p,
+                                       ctypes.c_size_t, ctypes.c_uint, ctypes.c_void_p]
+    L.gsn_msm_bases_destroy.argtypes = [ctypes.c_void_p]
+    L.gsn_msm_bases_info.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint),
+                                     ctypes.POINTER(ctypes.c_size_t)]
+    L.gsn_msm.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
     L.gsn_plan_execute.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
     L.gsn_plan_device_bytes.restype = ctypes.c_size_t
     L.gsn_plan_device_bytes.argtypes = [ctypes.c_void_p]
@@ -297,6 +303,46 @@ class Plan:
     def close(self):
         if getattr(self, "handle", None):
             lib().gsn_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ G1 MSM
+
+CURVE_MNT4753_G1, CURVE_MNT6753_G1 = 0, 1
+# (base field, scalar field) of each curve
+CURVE_FIELDS = {CURVE_MNT4753_G1: (FIELD_MNT6753_FR, FIELD_MNT4753_FR),
+                CURVE_MNT6753_G1: (FIELD_MNT4753_FR, FIELD_MNT6753_FR)}
+
+
+class MsmBases:
+    """Fixed bases prepared for repeated multi-scalar multiplications (the
+    curve form of the reference's multiexp, cuda/multi_exp.cu:140):
+    d_points holds n projective points (X, Y, Z), 3 x 24 u32 each."""
+
+    def __init__(self, curve, d_points, n, window_bits=0, stream=None):
+        h = ctypes.c_void_p()
+        check(lib().gsn_msm_bases_create(ctypes.byref(h), curve, ctypes.c_void_p(d_points), n, window_bits,
+                                         stream))
+        self.handle, self.curve, self.n = h, curve, n
+
+    def info(self):
+        c, b = ctypes.c_uint(), ctypes.c_size_t()
+        check(lib().gsn_msm_bases_info(self.handle, ctypes.byref(c), ctypes.byref(b)))
+        return c.value, b.value
+
+    def msm(self, d_out, d_scalars, stream=None):
+        """*d_out = sum_i [s_i] P_i (one projective point, device memory)."""
+        check(lib().gsn_msm(self.handle, ctypes.c_void_p(d_out), ctypes.c_void_p(d_scalars), stream))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().gsn_msm_bases_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -28,7 +28,7 @@ HIP_FLAGS = [
     "-I" + INCLUDE, "-I" + CSRC,
 ]
 HIP_SOURCES = ["kernels_mnt6.hip", "kernels_mnt4.hip", "kernels_fp32.hip", "gsn_runtime.hip",
-               "gsn_batch.hip", "gsn_domain.hip", "gsn_server.cpp", "gsn_staging.cpp"]
+               "gsn_batch.hip", "gsn_domain.hip", "gsn_server.cpp", "gsn_staging.cpp", "gsn_msm.hip"]
 CXX_SOURCES = ["best_fft.cpp", "multi_exp.cpp"]
 
 

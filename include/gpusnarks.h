@@ -194,6 +194,37 @@ int gsn_batch_execute(const gsn_batch_item *items, size_t count, void *stream);
 /* Kernel launches one replay of the batch issues. */
 int gsn_batch_num_launches(const gsn_batch *batch);
 
+/* --- multi-scalar multiplication on G1 ------------------------------------
+ * SWITCHED OFF in this build: gsn_msm_bases_create and gsn_msm return
+ * GSN_ERR_INVALID_ARGUMENT ("switched off") -- the first GPU run did not
+ * finish and its cause is not yet known.
+ *
+ * sum_i [s_i] P_i: the curve instantiation of the reference's multiexp
+ * (cuda/multi_exp.cu:140, mnt4753_G1 x Scalar), rebuilt as a bucket
+ * (Pippenger) method over a prepared table of fixed bases.  Points are
+ * homogeneous projective (X, Y, Z), 3 x 96 bytes, coordinates in Montgomery
+ * form over the curve's base field, Z = 0 the identity (the result is
+ * (0, 0, 0) then); scalars are canonical elements of the scalar field in
+ * Montgomery form.  The formulas are the reference's (add-1998-cmo-2 and
+ * dbl-2007-bl, cuda/device_field.h:335-378) over the base field, with the
+ * exceptional cases (P + P, P + (-P), identity) handled. */
+enum {
+    GSN_CURVE_MNT4753_G1 = 0, /* y^2 = x^3 + 2x + b over MNT6-753 Fr; scalars in MNT4-753 Fr */
+    GSN_CURVE_MNT6753_G1 = 1  /* y^2 = x^3 + 11x + b over MNT4-753 Fr; scalars in MNT6-753 Fr */
+};
+typedef struct gsn_msm_bases gsn_msm_bases;
+/* Prepares n bases (device pointer) for repeated MSMs: a table of
+ * [2^(c w)] P_i for every window of c scalar bits (window_bits 0: chosen
+ * from n), ceil(753 / c) * n * 288 bytes of device memory.  Asynchronous on
+ * `stream`; the bases may be freed once the stream has run. */
+int gsn_msm_bases_create(gsn_msm_bases **bases, int curve, const void *d_points, size_t n, unsigned window_bits,
+                         void *stream);
+int gsn_msm_bases_destroy(gsn_msm_bases *bases);
+int gsn_msm_bases_info(const gsn_msm_bases *bases, unsigned *window_bits, size_t *table_bytes);
+/* *d_out (one point, device memory) = sum_i [d_scalars[i]] P_i over the
+ * prepared bases; asynchronous on `stream` (scratch is stream-ordered). */
+int gsn_msm(gsn_msm_bases *bases, void *d_out, const void *d_scalars, void *stream);
+
 /* --- autotuning: per-size pass schedules -----------------------------------
  * A schedule is num_passes triples (log_r, log_w, log_e): pass radix 2^log_r,
  * 2^log_w columns per workgroup, register radix 2^log_e; the log_r sum to

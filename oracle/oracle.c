@@ -480,3 +480,147 @@ void or_extended_radix2_fft(const or_field *f, uint32_t *a, size_t n, const uint
         or_mul(f, sh_i, sh_i, shinv);
     }
 }
+
+/* ------------------------------------------------------------ G1 curves
+ * Short Weierstrass y^2 = x^3 + a x + b over the base field f, points in
+ * homogeneous projective coordinates (X : Y : Z), x = X/Z, y = Y/Z, all in
+ * Montgomery form; Z = 0 is the identity.  The reference's mnt4753_G1
+ * (cuda/device_field.h:296-437) uses these formulas -- addition
+ * "add-1998-cmo-2" (:335-352) and doubling "dbl-2007-bl" (:354-378) of the
+ * Explicit-Formulas Database -- but over the wrong field (its coordinates
+ * are Scalar, the curve's scalar field) and without the exceptional cases;
+ * here the coordinates are in the base field and P + P, P + (-P) and the
+ * identity are handled.  b never enters the formulas. */
+#define PT(p, k) ((p) + (size_t)(k) * (size_t)f->nlimbs)
+
+static int is_zero_el(const or_field *f, const uint32_t *a)
+{
+    for (int i = 0; i < f->nlimbs; ++i)
+        if (a[i])
+            return 0;
+    return 1;
+}
+
+void or_g1_dbl(const or_field *f, const uint32_t *coeff_a, uint32_t *r, const uint32_t *p)
+{
+    const int s = f->nlimbs;
+    const size_t eb = sizeof(uint32_t) * (size_t)s;
+    if (is_zero_el(f, PT(p, 2))) {
+        memmove(r, p, 3 * eb);
+        return;
+    }
+    uint32_t XX[OR_MAX_LIMBS], ZZ[OR_MAX_LIMBS], w[OR_MAX_LIMBS], t[OR_MAX_LIMBS], sv[OR_MAX_LIMBS],
+        ss[OR_MAX_LIMBS], sss[OR_MAX_LIMBS], R[OR_MAX_LIMBS], RR[OR_MAX_LIMBS], B[OR_MAX_LIMBS],
+        h[OR_MAX_LIMBS], X3[OR_MAX_LIMBS], Y3[OR_MAX_LIMBS];
+    const uint32_t *X = PT(p, 0), *Y = PT(p, 1), *Z = PT(p, 2);
+    or_mul(f, XX, X, X);
+    or_mul(f, ZZ, Z, Z);
+    or_mul(f, w, coeff_a, ZZ);          /* w = a ZZ + 3 XX */
+    or_add(f, w, w, XX);
+    or_add(f, w, w, XX);
+    or_add(f, w, w, XX);
+    or_mul(f, t, Y, Z);                 /* s = 2 Y Z */
+    or_add(f, sv, t, t);
+    or_mul(f, ss, sv, sv);
+    or_mul(f, sss, sv, ss);
+    or_mul(f, R, Y, sv);                /* R = Y s */
+    or_mul(f, RR, R, R);
+    or_add(f, t, X, R);                 /* B = (X + R)^2 - XX - RR */
+    or_mul(f, B, t, t);
+    or_sub(f, B, B, XX);
+    or_sub(f, B, B, RR);
+    or_mul(f, h, w, w);                 /* h = w^2 - 2 B */
+    or_sub(f, h, h, B);
+    or_sub(f, h, h, B);
+    or_mul(f, X3, h, sv);               /* X3 = h s */
+    or_sub(f, t, B, h);                 /* Y3 = w (B - h) - 2 RR */
+    or_mul(f, Y3, w, t);
+    or_sub(f, Y3, Y3, RR);
+    or_sub(f, Y3, Y3, RR);
+    memcpy(PT(r, 0), X3, eb);
+    memcpy(PT(r, 1), Y3, eb);
+    memcpy(PT(r, 2), sss, eb);          /* Z3 = s^3 */
+}
+
+void or_g1_add(const or_field *f, const uint32_t *coeff_a, uint32_t *r, const uint32_t *p, const uint32_t *q)
+{
+    const int s = f->nlimbs;
+    const size_t eb = sizeof(uint32_t) * (size_t)s;
+    if (is_zero_el(f, PT(p, 2))) {
+        memmove(r, q, 3 * eb);
+        return;
+    }
+    if (is_zero_el(f, PT(q, 2))) {
+        memmove(r, p, 3 * eb);
+        return;
+    }
+    const uint32_t *X1 = PT(p, 0), *Y1 = PT(p, 1), *Z1 = PT(p, 2);
+    const uint32_t *X2 = PT(q, 0), *Y2 = PT(q, 1), *Z2 = PT(q, 2);
+    uint32_t X1Z2[OR_MAX_LIMBS], Y1Z2[OR_MAX_LIMBS], Z1Z2[OR_MAX_LIMBS], u[OR_MAX_LIMBS], uu[OR_MAX_LIMBS],
+        v[OR_MAX_LIMBS], vv[OR_MAX_LIMBS], vvv[OR_MAX_LIMBS], R[OR_MAX_LIMBS], A[OR_MAX_LIMBS], t[OR_MAX_LIMBS],
+        X3[OR_MAX_LIMBS], Y3[OR_MAX_LIMBS], Z3[OR_MAX_LIMBS];
+    or_mul(f, X1Z2, X1, Z2);
+    or_mul(f, Y1Z2, Y1, Z2);
+    or_mul(f, Z1Z2, Z1, Z2);
+    or_mul(f, t, Y2, Z1);               /* u = Y2 Z1 - Y1 Z2 */
+    or_sub(f, u, t, Y1Z2);
+    or_mul(f, t, X2, Z1);               /* v = X2 Z1 - X1 Z2 */
+    or_sub(f, v, t, X1Z2);
+    if (is_zero_el(f, v)) {
+        if (is_zero_el(f, u)) {         /* P == Q */
+            or_g1_dbl(f, coeff_a, r, p);
+        } else {                        /* P == -Q */
+            memset(r, 0, 3 * eb);
+        }
+        return;
+    }
+    or_mul(f, uu, u, u);
+    or_mul(f, vv, v, v);
+    or_mul(f, vvv, vv, v);
+    or_mul(f, R, vv, X1Z2);
+    or_mul(f, A, uu, Z1Z2);             /* A = uu Z1Z2 - vvv - 2 R */
+    or_sub(f, A, A, vvv);
+    or_sub(f, A, A, R);
+    or_sub(f, A, A, R);
+    or_mul(f, X3, v, A);
+    or_sub(f, t, R, A);                 /* Y3 = u (R - A) - vvv Y1Z2 */
+    or_mul(f, Y3, u, t);
+    or_mul(f, t, vvv, Y1Z2);
+    or_sub(f, Y3, Y3, t);
+    or_mul(f, Z3, vvv, Z1Z2);
+    memcpy(PT(r, 0), X3, eb);
+    memcpy(PT(r, 1), Y3, eb);
+    memcpy(PT(r, 2), Z3, eb);
+}
+
+/* [k] P by double-and-add from the top bit; k has klimbs 32-bit limbs (a
+ * plain integer, not Montgomery form). */
+void or_g1_mul(const or_field *f, const uint32_t *coeff_a, uint32_t *r, const uint32_t *p, const uint32_t *k,
+               int klimbs)
+{
+    uint32_t acc[3 * OR_MAX_LIMBS];
+    memset(acc, 0, sizeof(acc));
+    for (int i = klimbs - 1; i >= 0; --i)
+        for (int b = 31; b >= 0; --b) {
+            or_g1_dbl(f, coeff_a, acc, acc);
+            if ((k[i] >> b) & 1u)
+                or_g1_add(f, coeff_a, acc, acc, p);
+        }
+    memcpy(r, acc, sizeof(uint32_t) * 3 * (size_t)f->nlimbs);
+}
+
+/* sum_i [k_i] P_i, the naive way (the reference's host multi_exp,
+ * test/multiexp.h:3-14, with a correct scalar multiplication); k_i plain
+ * integers of klimbs limbs each. */
+void or_g1_msm(const or_field *f, const uint32_t *coeff_a, uint32_t *r, const uint32_t *points,
+               const uint32_t *k, int klimbs, size_t n)
+{
+    const size_t ps = 3 * (size_t)f->nlimbs;
+    uint32_t acc[3 * OR_MAX_LIMBS], t[3 * OR_MAX_LIMBS];
+    memset(acc, 0, sizeof(acc));
+    for (size_t i = 0; i < n; ++i) {
+        or_g1_mul(f, coeff_a, t, points + i * ps, k + i * (size_t)klimbs, klimbs);
+        or_g1_add(f, coeff_a, acc, acc, t);
+    }
+    memcpy(r, acc, sizeof(uint32_t) * ps);
+}

@@ -84,6 +84,14 @@ void or_parallel_phase1_sample(const or_field *f, const uint32_t *a, size_t n,
 void or_inverse_fft(const or_field *f, uint32_t *a, size_t n, const uint32_t *omega);
 /* Extended radix-2 domain H u shift*H (|H| = n/2, omega_half generates H):
  * forward evaluates on H then shift*H; inverse interpolates back. */
+/* G1: projective points (X, Y, Z) of 3 * nlimbs limbs, Montgomery form, Z = 0
+ * the identity; coeff_a in Montgomery form; scalars plain integers. */
+void or_g1_dbl(const or_field *f, const uint32_t *coeff_a, uint32_t *r, const uint32_t *p);
+void or_g1_add(const or_field *f, const uint32_t *coeff_a, uint32_t *r, const uint32_t *p, const uint32_t *q);
+void or_g1_mul(const or_field *f, const uint32_t *coeff_a, uint32_t *r, const uint32_t *p, const uint32_t *k,
+               int klimbs);
+void or_g1_msm(const or_field *f, const uint32_t *coeff_a, uint32_t *r, const uint32_t *points,
+               const uint32_t *k, int klimbs, size_t n);
 void or_extended_radix2_fft(const or_field *f, uint32_t *a, size_t n, const uint32_t *omega_half,
                             const uint32_t *shift, int inverse);
 

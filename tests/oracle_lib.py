@@ -48,6 +48,10 @@ class Oracle:
         L.or_parallel_phase1_sample.argtypes = [ctypes.POINTER(OrField), _u32p, ctypes.c_size_t, _u32p,
                                                 ctypes.c_uint, ctypes.c_size_t, ctypes.c_size_t, _u32p, ctypes.c_int]
         L.or_inverse_fft.argtypes = [ctypes.POINTER(OrField), _u32p, ctypes.c_size_t, _u32p]
+        L.or_g1_add.argtypes = [ctypes.POINTER(OrField), _u32p, _u32p, _u32p, _u32p]
+        L.or_g1_dbl.argtypes = [ctypes.POINTER(OrField), _u32p, _u32p, _u32p]
+        L.or_g1_mul.argtypes = [ctypes.POINTER(OrField), _u32p, _u32p, _u32p, _u32p, ctypes.c_int]
+        L.or_g1_msm.argtypes = [ctypes.POINTER(OrField), _u32p, _u32p, _u32p, _u32p, ctypes.c_int, ctypes.c_size_t]
         L.or_extended_radix2_fft.argtypes = [ctypes.POINTER(OrField), _u32p, ctypes.c_size_t, _u32p, _u32p,
                                              ctypes.c_int]
         self.lib = L
@@ -113,6 +117,35 @@ class Oracle:
                                         _p(np.ascontiguousarray(omega_half, np.uint32)),
                                         _p(np.ascontiguousarray(shift, np.uint32)), int(inverse))
         return a
+
+    # G1 points: (3, 24) uint32 arrays (X, Y, Z), Montgomery form over the
+    # base field `fid`; coeff_a one Montgomery-form element
+    def g1_add(self, fid, coeff_a, p, q):
+        r = np.zeros((3, 24), np.uint32)
+        self.lib.or_g1_add(self.field(fid), _p(np.ascontiguousarray(coeff_a, np.uint32)), _p(r),
+                           _p(np.ascontiguousarray(p, np.uint32)), _p(np.ascontiguousarray(q, np.uint32)))
+        return r
+
+    def g1_dbl(self, fid, coeff_a, p):
+        r = np.zeros((3, 24), np.uint32)
+        self.lib.or_g1_dbl(self.field(fid), _p(np.ascontiguousarray(coeff_a, np.uint32)), _p(r),
+                           _p(np.ascontiguousarray(p, np.uint32)))
+        return r
+
+    def g1_mul(self, fid, coeff_a, p, k_limbs):
+        r = np.zeros((3, 24), np.uint32)
+        k = np.ascontiguousarray(k_limbs, np.uint32)
+        self.lib.or_g1_mul(self.field(fid), _p(np.ascontiguousarray(coeff_a, np.uint32)), _p(r),
+                           _p(np.ascontiguousarray(p, np.uint32)), _p(k), k.shape[-1])
+        return r
+
+    def g1_msm(self, fid, coeff_a, points, k_limbs):
+        r = np.zeros((3, 24), np.uint32)
+        pts = np.ascontiguousarray(points, np.uint32)
+        k = np.ascontiguousarray(k_limbs, np.uint32)
+        self.lib.or_g1_msm(self.field(fid), _p(np.ascontiguousarray(coeff_a, np.uint32)), _p(r), _p(pts), _p(k),
+                           k.shape[-1], pts.shape[0])
+        return r
 
     def inverse_fft(self, fid, x, omega):
         a = np.ascontiguousarray(x, np.uint32).copy()

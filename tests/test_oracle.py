@@ -268,3 +268,65 @@ def test_extended_domain_mnt4_full_size(orc):
         assert synth.to_int(out[k]) * rinv % p == _eval_poly(cv, pow(w, k, p), p)
         assert synth.to_int(out[m + k]) * rinv % p == _eval_poly(cv, 13 * pow(w, k, p) % p, p)
     assert np.array_equal(orc.extended_fft(fid, out, mont(w), mont(13), inverse=True), c)
+
+
+# ---------------------------------------------------------------- G1 curves
+from tests import g1_affine as aff  # noqa: E402
+
+# (name, base-field id, coeff a, scalar-field id): MNT4-753 G1 lies over
+# MNT6-753 Fr (= MNT4-753 Fq) with a = 2, MNT6-753 G1 over MNT4-753 Fr with
+# a = 11 (the reference's coeff_a, cuda/device_field.h:300)
+G1_CURVES = [("mnt4753_g1", 2, 2, 1), ("mnt6753_g1", 1, 11, 2)]
+
+
+def _g1_setup(orc, fid, a, seed):
+    q = synth.to_int(orc.params(fid)["mod"])
+    R = 1 << 768
+    b, P0 = aff.curve_with_point(q, a, seed)
+    a_m = synth.from_int(a * R % q, 24)
+    return q, R, b, P0, a_m
+
+
+@pytest.mark.parametrize("name,fid,a,sfid", G1_CURVES)
+def test_g1_group_law_matches_affine(orc, name, fid, a, sfid):
+    """The oracle's projective add / dbl / [k]P / MSM against textbook affine
+    arithmetic, including P + P, P + (-P), the identity, and results
+    re-checked on the curve."""
+    q, R, b, P0, a_m = _g1_setup(orc, fid, a, 0x61 + fid)
+    Ps = [aff.mul(k, P0, a, q) for k in (1, 2, 3, 12345, 2 ** 200 + 7)]
+    proj = [aff.to_projective(P, q, R) for P in Ps]
+    # scale one representative: (X, Y, Z) and (cX, cY, cZ) are the same point
+    c = 0xC0FFEE
+    proj[3] = orc.mul(fid, proj[3], np.tile(synth.from_int(c * R % q, 24), (3, 1)))
+    for i in range(len(Ps)):
+        for j in range(len(Ps)):
+            got = aff.to_affine(orc.g1_add(fid, a_m, proj[i], proj[j]), q, R)
+            want = aff.add(Ps[i], Ps[j], a, q)
+            assert got == want and aff.on_curve(got, a, b, q), (i, j)
+        assert aff.to_affine(orc.g1_dbl(fid, a_m, proj[i]), q, R) == aff.add(Ps[i], Ps[i], a, q)
+    neg = proj[2].copy()
+    neg[1] = orc.sub(fid, np.zeros((1, 24), np.uint32), neg[1:2])[0]
+    assert aff.to_affine(orc.g1_add(fid, a_m, proj[2], neg), q, R) is aff.INF
+    zero = np.zeros((3, 24), np.uint32)
+    assert aff.to_affine(orc.g1_add(fid, a_m, zero, proj[1]), q, R) == Ps[1]
+    assert aff.to_affine(orc.g1_add(fid, a_m, zero, zero), q, R) is aff.INF
+    r = synth.to_int(orc.params(sfid)["mod"])
+    for k in (0, 1, 2, 3, 5, r - 1, 0x1234567890ABCDEF1234567890ABCDEF % r):
+        got = aff.to_affine(orc.g1_mul(fid, a_m, proj[0], synth.from_int(k, 24)), q, R)
+        assert got == aff.mul(k, Ps[0], a, q), k
+
+
+@pytest.mark.parametrize("name,fid,a,sfid", G1_CURVES)
+def test_g1_msm_matches_affine(orc, name, fid, a, sfid):
+    q, R, b, P0, a_m = _g1_setup(orc, fid, a, 0x71 + fid)
+    r = synth.to_int(orc.params(sfid)["mod"])
+    import random
+    rng = random.Random(5)
+    ks = [rng.randrange(r) for _ in range(5)] + [0]
+    Ps = [aff.mul(rng.randrange(1, 2 ** 64), P0, a, q) for _ in range(6)]
+    want = aff.INF
+    for k, P in zip(ks, Ps):
+        want = aff.add(want, aff.mul(k, P, a, q), a, q)
+    pts = np.stack([aff.to_projective(P, q, R) for P in Ps])
+    got = orc.g1_msm(fid, a_m, pts, np.stack([synth.from_int(k, 24) for k in ks]))
+    assert aff.to_affine(got, q, R) == want
