@@ -30,6 +30,7 @@
 #include <tuple>
 #include <vector>
 
+#include "batch_impl.h"
 #include "gpusnarks.h"
 #include "gsn_internal.h"
 #include "plan_impl.h"
@@ -339,8 +340,11 @@ template <>
 fields::ScalarMNT6 default_shift<GSN_FIELD_MNT6753_FR>() { return fields::ScalarMNT6(Mnt6753FrParams::QNR); }
 
 // A transform over the extended domain H u g*H of n = 2m points: one
-// elementwise kernel and two size-m plans (forward: combine, then
-// transform both halves; inverse: transform both halves, then solve).
+// elementwise kernel and two size-m transforms (forward: combine, then
+// transform both halves; inverse: transform both halves, then solve).  The
+// two halves run as one batch (each pass level of both in one fused
+// launch), built for the buffers of the first call and rebuilt when they
+// change.
 template <int FID>
 struct ExtPlan : PlanBase {
     using H = typename FieldInfo<FID>::Host;
@@ -348,7 +352,27 @@ struct ExtPlan : PlanBase {
     PlanBase *half = nullptr;
     CosetTables *tab = nullptr;  // g^i (forward) or g^-i (inverse), i < m
     typename F::tw_t c;          // G (forward) or 1 / (1 - G) (inverse)
+    std::unique_ptr<BatchImpl> halves;  // both half transforms, fused
+    const void *halves_in = nullptr;
+    void *halves_out = nullptr;
     ~ExtPlan() override { delete half; }
+
+    // both size-m transforms of in[0, m), in[m, 2m) into out[0, m), out[m, 2m)
+    int run_halves(const void *in, void *out, hipStream_t st)
+    {
+        if (!halves || halves_in != in || halves_out != out) {
+            const size_t hb = ((size_t)1 << (log_n - 1)) * sizeof(typename F::elem_t);
+            const gsn_batch_item items[2] = {
+                {FID, log_n - 1, inverse, nullptr, in, out},
+                {FID, log_n - 1, inverse, nullptr, (const char *)in + hb, (char *)out + hb}};
+            std::unique_ptr<BatchImpl> b(new BatchImpl());
+            GSN_TRY(build_batch(*b, items, 2, 0));
+            halves = std::move(b);
+            halves_in = in;
+            halves_out = out;
+        }
+        return halves->enqueue(st);
+    }
 
     int init(unsigned L, const H &g, int inv)
     {
@@ -362,7 +386,7 @@ struct ExtPlan : PlanBase {
         GSN_TRY(create_plan(&half, FID, lm, nullptr, inv));
         GSN_TRY(coset_tables<FID>(&tab, g, lm, inv));
         c = inv ? record<FID>((H::one() - G).mul_inv()) : record<FID>(G);
-        num_passes = 2 * half->num_passes + 1;
+        num_passes = half->num_passes + 1;  // the halves' passes are fused
         device_bytes = half->device_bytes;
         return GSN_OK;
     }
@@ -371,18 +395,15 @@ struct ExtPlan : PlanBase {
     {
         using E = typename F::elem_t;
         const size_t m = (size_t)1 << (log_n - 1);
-        const size_t hb = m * sizeof(E);
         const dim3 grid(blocks_for(m)), block(256);
         const auto *lo = (const typename F::tw_t *)tab->lo.ptr, *hi = (const typename F::tw_t *)tab->hi.ptr;
         if (!inverse) {
             GSN_TRY(take_injected_fault());
             hipLaunchKernelGGL(ext_fwd_kernel<F>, grid, block, 0, st, (E *)out, (const E *)in, lo, hi, tab->h, c, m);
             GSN_TRY(hip_status(hipGetLastError(), "extended-domain launch"));
-            GSN_TRY(half->execute(out, out, st));
-            return half->execute((char *)out + hb, (char *)out + hb, st);
+            return run_halves(out, out, st);
         }
-        GSN_TRY(half->execute(in, out, st));
-        GSN_TRY(half->execute((const char *)in + hb, (char *)out + hb, st));
+        GSN_TRY(run_halves(in, out, st));
         GSN_TRY(take_injected_fault());
         hipLaunchKernelGGL(ext_inv_kernel<F>, grid, block, 0, st, (E *)out, lo, hi, tab->h, c, m);
         return hip_status(hipGetLastError(), "extended-domain launch");

@@ -87,6 +87,29 @@ def test_roundtrip_and_linearity_full_size(field, log_n):
         assert (g.best_fft(d, None, field) == one).all()
 
 
+def test_fp32_largest_configured_size():
+    """n = 2^28 (the largest 32-bit configuration, 1 GiB) on device buffers:
+    forward then inverse returns the input, and X_0 = sum of the inputs."""
+    field, log_n = g.FIELD_FP32, 28
+    p = g.MODULI[field]
+    say("fp32 2^28 round trip")
+    # a multiplicative hash of the index, reduced mod p (uniform enough, and
+    # cheap at 2^28)
+    x = ((np.arange(1 << log_n, dtype=np.uint64) * 2654435761 + 0x2828) % p).astype(np.uint32).reshape(-1, 1)
+    d = g.DeviceArray.from_numpy(x)
+    e = g.DeviceArray(x.nbytes)
+    fwd, inv = g.Plan(field, log_n), g.Plan(field, log_n, inverse=True)
+    fwd.execute(d.ptr, e.ptr)
+    g.synchronize()
+    X0 = e.download((1, 1))[0, 0]
+    # the inputs are Montgomery forms; the transform is linear, so X_0 is
+    # their sum mod p in the same representation
+    assert int(X0) == int(x[:, 0].astype(np.uint64).sum() % p)
+    inv.execute(e.ptr, e.ptr)
+    g.synchronize()
+    assert np.array_equal(e.download(x.shape), x)
+
+
 @pytest.mark.parametrize("field,log_n", [(g.FIELD_MNT6753_FR, 17), (g.FIELD_FP32, 23), (g.FIELD_FP32, 21)])
 def test_host_entry_staging_matches_device_path(field, log_n):
     """The host entry point moves buffers above one 8 MiB staging chunk
