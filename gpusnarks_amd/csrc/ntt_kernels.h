@@ -201,8 +201,16 @@ __device__ __forceinline__ void pass32_body(const PassArgs<F32> &a, uint32_t blk
                 return ldtw32(a.tw_pass, ex);
             return tw32_product(ldtw32(a.tw_pass, ex >> lb), ldtw32(a.tw_pass_lo, ex & lm));
         };
-        const F::tw_t tB = twx((uint32_t)(R / E) * k), tC = twx((uint32_t)(R / E0) * k);
+        // B and C depend on the column only: formed once per tile and column
+        // (2W lookups instead of two per thread)
+        __shared__ F::tw_t colBC[2 * W];
+        for (int i = t; i < 2 * W; i += NT) {
+            const uint32_t kc = (j0 + (uint32_t)(i & (W - 1))) & ns_mask;
+            colBC[i] = twx((i < W ? (uint32_t)(R / E) : (uint32_t)(R / E0)) * kc);
+        }
         F::tw_t tg = twx((uint32_t)jj * k);
+        __syncthreads();
+        const F::tw_t tB = colBC[c], tC = colBC[W + c];
 #pragma unroll
         for (int g = 0; g < G0; ++g) {
             F::tw_t te = tg;
