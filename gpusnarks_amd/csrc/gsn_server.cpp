@@ -123,6 +123,45 @@ void *wave_alloc(void *ctx, size_t bytes)
     return p;
 }
 
+// One request alone: its plan's passes launched directly, with PassArgs by
+// value and the wave's own workspace -- no job tables to upload, which a
+// fused batch needs.
+template <int FID>
+int enqueue_single_t(PlanBase *base, const gsn_batch_item &it, Wave *w, hipStream_t s)
+{
+    using F = typename FieldInfo<FID>::Dev;
+    const PlanImpl<FID> &plan = *static_cast<PlanImpl<FID> *>(base);
+    const size_t n = (size_t)1 << it.log_n;
+    if (plan.cfg.empty()) {  // n == 1: the canonical copy
+        launch_scale<F>((const typename F::elem_t *)it.d_in, (typename F::elem_t *)it.d_out, {}, 1, 0, s);
+        return hip_status(hipGetLastError(), "copy launch");
+    }
+    const int P = plan.num_passes;
+    void *ws0 = nullptr, *ws1 = nullptr;
+    if (P >= 2 && !(ws0 = wave_alloc(w, n * sizeof(typename F::elem_t))))
+        return set_error(GSN_ERR_OUT_OF_MEMORY, "wave memory exhausted");
+    if (P >= 3 && !(ws1 = wave_alloc(w, n * sizeof(typename F::elem_t))))
+        return set_error(GSN_ERR_OUT_OF_MEMORY, "wave memory exhausted");
+    std::vector<PassJob<F>> jobs;
+    plan.make_jobs(it.d_in, it.d_out, ws0, ws1, jobs);
+    for (size_t i = 0; i < jobs.size(); ++i) {
+        launch_pass<F>(jobs[i], plan.cfg[i], s);
+        GSN_TRY(hip_status(hipGetLastError(), "ntt pass launch"));
+    }
+    return GSN_OK;
+}
+
+int enqueue_single(const gsn_batch_item &it, Wave *w, hipStream_t s)
+{
+    PlanBase *p = nullptr;
+    GSN_TRY(cached_plan(&p, it.field, it.log_n, it.omega, it.inverse));
+    switch (it.field) {
+    case GSN_FIELD_FP32: return enqueue_single_t<GSN_FIELD_FP32>(p, it, w, s);
+    case GSN_FIELD_MNT4753_FR: return enqueue_single_t<GSN_FIELD_MNT4753_FR>(p, it, w, s);
+    default: return enqueue_single_t<GSN_FIELD_MNT6753_FR>(p, it, w, s);
+    }
+}
+
 // Device bytes a request needs in a wave: operand (host requests) +
 // workspace + job tables, each rounded to the allocator alignment.
 size_t request_bytes(int field, unsigned log_n, bool device)
@@ -425,16 +464,18 @@ class Server {
                 it.d_out = d;
             }
         }
-        if (st == GSN_OK) {
+        if (st == GSN_OK && items.size() == 1) {
+            st = enqueue_single(items[0], w.get(), s);
+        } else if (st == GSN_OK) {
             w->batch.reset(new BatchImpl());
             BatchMemory mem;
             mem.alloc = wave_alloc;
             mem.ctx = w.get();
             mem.stream = s;
             st = build_batch(*w->batch, items.data(), items.size(), 0, mem);
+            if (st == GSN_OK)
+                st = w->batch->enqueue(s);
         }
-        if (st == GSN_OK)
-            st = w->batch->enqueue(s);
         pin_cursor = w->pin_off;
         for (size_t i = 0; i < w->reqs.size() && st == GSN_OK; ++i) {
             Request *r = w->reqs[i];
