@@ -65,7 +65,7 @@ def test_batch_matches_solo(use_graph):
 
 
 def test_batch_in_place_and_large_mix():
-    # the north-star ragged mix at small scale: 753-bit 2^10..2^16, Fp32 2^10..2^18
+    # the ragged mix of the batched workload at small scale: 753-bit 2^10..2^16, Fp32 2^10..2^18
     items = ragged_items(5, 24, sizes753=(10, 16), sizes32=(10, 18), mnt4=False)
     want = [solo(f, x, inv) for f, _, inv, x in items]
     d = [g.DeviceArray.from_numpy(x) for _, _, _, x in items]

@@ -28,9 +28,10 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_cpp_dropin_symbols_exported():
-    # best_fft / best_ifft explicit instantiations (mangled C++ names)
+    # best_fft / best_ifft / multiexp explicit instantiations (mangled C++ names)
     out = os.popen("nm -D --defined-only %s" % g.library_path).read()
-    for name in ("best_fftIN6fields10PrimeField", "best_ifftIN6fields10PrimeField"):
+    for name in ("best_fftIN6fields10PrimeField", "best_ifftIN6fields10PrimeField",
+                 "multiexpIN6fields10PrimeField"):
         assert name in out
 
 
