@@ -124,3 +124,22 @@ def test_host_entry_rejects_non_canonical_input():
         with pytest.raises(g.GsnError) as e:
             g.best_fft(x, None, field)
         assert e.value.status == 1 and "not below p" in g.lib().gsn_last_error().decode()
+
+
+def test_missing_library_fails_loudly(monkeypatch):
+    """No CPU fallback: without libgpusnarks.so the package refuses to run."""
+    monkeypatch.setattr(g, "library_path", "/nonexistent/libgpusnarks.so")
+    monkeypatch.setattr(g, "_lib", None)
+    with pytest.raises(ImportError):
+        g.lib()
+
+
+def test_transform_without_device_is_an_error():
+    """On a host with no GPU the transform returns GSN_ERR_NO_DEVICE rather
+    than computing anything on the CPU."""
+    if g.lib().gsn_device_init(0) == 0:
+        pytest.skip("a GPU is present")
+    x = np.zeros((16, 1), np.uint32)
+    with pytest.raises(g.GsnError) as e:
+        g.best_fft(x, None, g.FIELD_FP32)
+    assert e.value.status == 5  # GSN_ERR_NO_DEVICE
