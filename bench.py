@@ -286,6 +286,32 @@ def bench_configs(threads, skip_cpu):
                      "elements_per_s": (1 << 16) / (ms * 1e-3),
                      "frac_of_mad_peak": bfly(16) * MADS_PER_BFLY_753 / (ms * 1e-3) / MAD_PEAK})
         log("config mnt4 2^16 extended%s: %.4f ms" % (" inverse" if inv else "", ms))
+    # MNT4-753 Fr past 2^16: n = 525 * 2^15 over the mixed-radix plan (odd
+    # DFTs of 525 points in LDS, 525 radix-2 transforms of 2^15, a transpose)
+    import math
+    n_mix = 525 << 15
+    rng = np.random.default_rng(0x5EED0011)
+    x = rng.integers(0, 1 << 32, size=(n_mix, 24), dtype=np.uint64).astype(np.uint32)
+    x[:, 23] %= np.uint32(g.MODULI[g.FIELD_MNT4753_FR] >> (32 * 23))
+    d = g.DeviceArray.from_numpy(x)
+    for inv in (False, True):
+        plan = g.Plan.mixed(g.FIELD_MNT4753_FR, n_mix, inverse=inv)
+        s = g.Stream()
+        plan.execute(d.ptr, d.ptr, s.handle)
+        e0, e1 = g.Event(), g.Event()
+        e0.record(s.handle)
+        for _ in range(5):
+            plan.execute(d.ptr, d.ptr, s.handle)
+        e1.record(s.handle)
+        e1.synchronize()
+        ms = g.Event.elapsed_ms(e0, e1) / 5
+        rows.append({"field": "mnt4", "n": n_mix, "log2_n": math.log2(n_mix), "inverse": inv,
+                     "domain": "mixed radix 525 * 2^15", "ms": ms, "passes": plan.num_passes,
+                     "elements_per_s": n_mix / (ms * 1e-3),
+                     "radix2_equivalent_butterflies_per_s": n_mix / 2 * math.log2(n_mix) / (ms * 1e-3)})
+        log("config mnt4 525*2^15 mixed%s: %.3f ms" % (" inverse" if inv else "", ms))
+        plan.close()
+    d.free()
     out = {"plain": rows}
     if not skip_cpu:
         out["cpu_fp32"] = [cpu_baseline(l, threads, g.FIELD_FP32, serial=(l == 20)) for l in (16, 20, 22)]

@@ -80,6 +80,8 @@ def lib():
                                   ctypes.c_void_p, ctypes.c_int]
     L.gsn_plan_create_ext.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_uint,
                                       ctypes.c_void_p, ctypes.c_int]
+    L.gsn_plan_create_mixed.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_size_t,
+                                        ctypes.c_int]
     L.gsn_plan_destroy.argtypes = [ctypes.c_void_p]
     L.gsn_msm_bases_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_void_p,
                                        ctypes.c_size_t, ctypes.c_uint, ctypes.c_void_p]
@@ -274,9 +276,21 @@ class Plan:
         self.handle = h
         return self
 
+    @classmethod
+    def mixed(cls, field, n, inverse=False):
+        """Plan for n = n1 * 2^a points (n1 > 1 odd, from the 3/5/7-smooth part
+        of p - 1) over w = w_{2^a} * ROOT_ODD^(ODD_MAX / n1)."""
+        self = cls.__new__(cls)
+        self.field, self.inverse, self._omega = field, bool(inverse), None
+        self.log_n, self._n = None, n
+        h = ctypes.c_void_p()
+        check(lib().gsn_plan_create_mixed(ctypes.byref(h), field, n, int(self.inverse)))
+        self.handle = h
+        return self
+
     @property
     def n(self):
-        return 1 << self.log_n
+        return getattr(self, "_n", None) or 1 << self.log_n
 
     @property
     def device_bytes(self):

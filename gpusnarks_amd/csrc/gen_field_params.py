@@ -82,6 +82,25 @@ def smallest_qnr(p):
     return g
 
 
+def odd_smooth_root(p):
+    """(M, root): M = the largest divisor of p - 1 made of the primes 3, 5, 7,
+    and a primitive M-th root of unity (g^((p-1)/M) for the smallest g that
+    is no l-th power for any such prime l dividing M)."""
+    q, m, primes = p - 1, 1, []
+    for l in (3, 5, 7):
+        while q % l == 0:
+            q //= l
+            m *= l
+            if l not in primes:
+                primes.append(l)
+    g = 2
+    while any(pow(g, (p - 1) // l, p) == 1 for l in primes):
+        g += 1
+    root = pow(g, (p - 1) // m, p)
+    assert pow(root, m, p) == 1 and all(pow(root, m // l, p) != 1 for l in primes)
+    return m, root
+
+
 def limbs(x, bits, count):
     mask = (1 << bits) - 1
     out = []
@@ -115,6 +134,8 @@ def field753(name, p):
         root_api=limbs(root * r_api % p, 32, 24),        # primitive 2^s-th root, API Montgomery form
         root=root,
     )
+    d["odd_max"], odd_root = odd_smooth_root(p)
+    d["root_odd_api"] = limbs(odd_root * r_api % p, 32, 24)
     return d
 
 
@@ -150,6 +171,10 @@ def main():
     L.append("  static constexpr uint32_t GENERATOR = %du;" % g32)
     L.append("  static constexpr uint32_t ROOT_MONT = 0x%08xu;     // primitive 2^30-th root, Montgomery form"
              % (root32 * (1 << 32) % FP32))
+    odd32, oroot32 = odd_smooth_root(FP32)
+    L.append("  static constexpr uint32_t ODD_MAX = %du;  // 3/5/7-smooth part of p - 1" % odd32)
+    L.append("  static constexpr uint32_t ROOT_ODD = 0x%08xu;   // primitive ODD_MAX-th root, Montgomery form"
+             % (oroot32 * (1 << 32) % FP32))
     L.append("};")
     L.append("")
     for f in (f4, f6):
@@ -169,6 +194,9 @@ def main():
                  % carr(limbs(f["pinv783"], 29, 27)))
         L.append("  static constexpr uint32_t ROOT_MONT[24] = %s;  // primitive 2^%d-th root, Montgomery form"
                  % (carr(f["root_api"]), f["s"]))
+        L.append("  static constexpr uint32_t ODD_MAX = %du;  // 3/5/7-smooth part of p - 1" % f["odd_max"])
+        L.append("  static constexpr uint32_t ROOT_ODD[24] = %s;  // primitive ODD_MAX-th root, Montgomery form"
+                 % carr(f["root_odd_api"]))
         L.append("  static constexpr uint32_t TWO_ADICITY = %du;" % f["s"])
         L.append("  static constexpr uint32_t QNR = %du;" % f["g"])
         L.append("};")

@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -195,6 +196,78 @@ __global__ __launch_bounds__(256) void ext_inv_kernel(typename F::elem_t *a, con
     const typename F::elem_t c_hi = F::canon(F::mul(F::sub_mod(u, v), K));
     F::store(a, i, F::sub_mod(u, c_hi));
     F::store(a, i + m, c_hi);
+}
+
+// Mixed-radix transforms, n = n1 * n2 with n1 odd (3, 5, 7 factors of p - 1)
+// and n2 = 2^a, by the four-step factorisation (j = n2 j1 + j2, k = k1 + n1 k2):
+//   Y[k1][j2] = w^(j2 k1) sum_j1 x[n2 j1 + j2] w^(n2 j1 k1)   (mixed_odd_kernel)
+//   Z[k1][k2] = sum_j2 Y[k1][j2] w^(n1 j2 k2)                  (radix-2 plans on rows)
+//   X[k1 + n1 k2] = Z[k1][k2]                                   (mixed_transpose_kernel)
+// The odd DFT of one column runs in LDS as radix-3/5/7 Stockham passes: a
+// pass of radix q over stride L twiddles input r of group j by
+// w_{Lq}^(r (j mod L)) and writes output s to (j / L) L q + (j mod L) + s L.
+// Products are general Montgomery products against tables of Montgomery
+// forms: T[e] = (w^n2)^e for e < n1, and w^e = lo[e mod 2^h] hi[e >> h].
+constexpr int MIX_THREADS = 128;
+struct MixRadices {
+    uint32_t count;
+    uint32_t q[16];
+};
+
+template <class O, int MAXN1>
+__global__ __launch_bounds__(MIX_THREADS) void mixed_odd_kernel(typename O::elem *ws, const typename O::elem *in,
+                                                                uint32_t n1, uint32_t n2, const typename O::elem *T,
+                                                                const typename O::elem *lo,
+                                                                const typename O::elem *hi, uint32_t h,
+                                                                MixRadices rad)
+{
+    using E = typename O::elem;
+    __shared__ E buf[2 * MAXN1];
+    E *A = buf, *B = buf + MAXN1;
+    const uint32_t j2 = blockIdx.x;
+    for (uint32_t j1 = threadIdx.x; j1 < n1; j1 += MIX_THREADS)
+        A[j1] = O::load(in, (size_t)n2 * j1 + j2);
+    __syncthreads();
+    uint32_t L = 1;
+    for (uint32_t pi = 0; pi < rad.count; ++pi) {
+        const uint32_t q = rad.q[pi], nq = n1 / q, span = n1 / (L * q);
+        for (uint32_t j = threadIdx.x; j < nq; j += MIX_THREADS) {
+            const uint32_t k = j % L, jb = j / L;
+            if (k != 0)  // r k span < n1: w_{Lq}^(r k) = T[r k span]
+                for (uint32_t r = 1; r < q; ++r)
+                    A[j + r * nq] = O::mul(A[j + r * nq], O::load(T, (size_t)r * k * span));
+            for (uint32_t sx = 0; sx < q; ++sx) {
+                E acc = A[j];
+                for (uint32_t r = 1; r < q; ++r)
+                    acc = O::add(acc, sx == 0 ? A[j + r * nq] : O::mul(A[j + r * nq], O::load(T, (size_t)((r * sx) % q) * nq)));
+                B[jb * L * q + k + sx * L] = acc;
+            }
+        }
+        __syncthreads();
+        E *t = A;
+        A = B;
+        B = t;
+        L *= q;
+    }
+    // A holds the column's DFT; twiddle by w^(j2 k1) (j2 k1 < n) and store
+    // row-major by k1
+    for (uint32_t k1 = threadIdx.x; k1 < n1; k1 += MIX_THREADS) {
+        const uint64_t e = (uint64_t)j2 * k1;
+        const E v = O::mul(O::mul(A[k1], O::load(lo, (size_t)(e & ((1ull << h) - 1)))), O::load(hi, (size_t)(e >> h)));
+        O::store(ws, (size_t)k1 * n2 + j2, v);
+    }
+}
+
+// out[k1 + n1 k2] = ws[k1 n2 + k2]
+template <class O>
+__global__ __launch_bounds__(256) void mixed_transpose_kernel(typename O::elem *out, const typename O::elem *ws,
+                                                             uint32_t n1, uint32_t log_n2, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n)
+        return;
+    const size_t k1 = i >> log_n2, k2 = i & (((size_t)1 << log_n2) - 1);
+    O::store(out, k1 + (size_t)n1 * k2, O::load(ws, i));
 }
 
 // ------------------------------------------------------------ host side
@@ -410,6 +483,146 @@ struct ExtPlan : PlanBase {
     }
 };
 
+// ---- mixed radix
+
+template <int FID>
+struct MixInfo;
+template <>
+struct MixInfo<GSN_FIELD_FP32> {
+    using O = Ops32;
+    static constexpr int MAXN1 = 3;
+    static constexpr uint32_t ODD_MAX = Fp32Params::ODD_MAX;
+    static fields::Fp32 root_odd() { return fields::Fp32::from_raw(&Fp32Params::ROOT_ODD); }
+};
+template <>
+struct MixInfo<GSN_FIELD_MNT4753_FR> {
+    using O = Ops753<Mnt4753FrParams>;
+    static constexpr int MAXN1 = 525;
+    static constexpr uint32_t ODD_MAX = Mnt4753FrParams::ODD_MAX;
+    static fields::Scalar root_odd() { return fields::Scalar::from_raw(Mnt4753FrParams::ROOT_ODD); }
+};
+template <>
+struct MixInfo<GSN_FIELD_MNT6753_FR> {
+    using O = Ops753<Mnt6753FrParams>;
+    static constexpr int MAXN1 = 735;  // two LDS buffers of 96-byte elements within 160 KB
+    static constexpr uint32_t ODD_MAX = Mnt6753FrParams::ODD_MAX;
+    static fields::ScalarMNT6 root_odd() { return fields::ScalarMNT6::from_raw(Mnt6753FrParams::ROOT_ODD); }
+};
+
+// A transform of n = n1 2^a points (n1 > 1 odd, dividing the 3/5/7-smooth
+// part of p - 1) over the primitive n-th root w = w_{2^a} w_{n1} (the
+// standard 2^a-th root times ROOT_ODD^(ODD_MAX / n1)): the odd-DFT kernel
+// into the plan's workspace, n1 radix-2 transforms of its rows as one
+// batch, and the transpose into the output.
+template <int FID>
+struct MixedPlan : PlanBase {
+    using H = typename FieldInfo<FID>::Host;
+    using MI = MixInfo<FID>;
+    using O = typename MI::O;
+    using E = typename O::elem;
+    uint32_t n1 = 0, a = 0, h = 0;
+    MixRadices rad{};
+    DeviceBuffer T, lo, hi, ws;
+    std::unique_ptr<BatchImpl> rows;
+
+    static void put(std::vector<E> &v, size_t i, const H &x)
+    {
+        std::memcpy(&v[i], x.im_rep, sizeof(E));
+    }
+
+    int init(uint32_t n1_, uint32_t a_, int inv)
+    {
+        field = FID;
+        inverse = inv;
+        n1 = n1_;
+        a = a_;
+        const size_t n2 = (size_t)1 << a, n = (size_t)n1 * n2;
+        log_n = 0;  // not a power of two; the size is n1 2^a
+        uint32_t m = n1;
+        for (uint32_t l : {7u, 5u, 3u})
+            for (; m % l == 0; m /= l)
+                rad.q[rad.count++] = l;
+        const H w2 = root_of_unity(FieldInfo<FID>::root_max(), FieldInfo<FID>::TWO_ADICITY, a);
+        const H w = w2 * MI::root_odd().pow(MI::ODD_MAX / n1);
+        const H base = inv ? w.mul_inv() : w;
+        // T[e] = (base^n2)^e, e < n1
+        std::vector<E> t(n1);
+        const H w1 = base.pow(n2);
+        H acc = H::one();
+        for (uint32_t e = 0; e < n1; ++e, acc = acc * w1)
+            put(t, e, acc);
+        // base^e = lo[e mod 2^h] hi[e >> h] (hi scaled by n1^-1 for the inverse)
+        while (((size_t)1 << (2 * h)) < n)
+            ++h;
+        const size_t nlo = (size_t)1 << h, nhi = (n + nlo - 1) >> h;
+        std::vector<E> vlo(nlo), vhi(nhi);
+        acc = H::one();
+        for (size_t e = 0; e < nlo; ++e, acc = acc * base)
+            put(vlo, e, acc);
+        H hacc = inv ? H(n1).mul_inv() : H::one();
+        for (size_t y = 0; y < nhi; ++y, hacc = hacc * acc)
+            put(vhi, y, hacc);
+        GSN_TRY(T.alloc(n1 * sizeof(E)));
+        GSN_TRY(lo.alloc(nlo * sizeof(E)));
+        GSN_TRY(hi.alloc(nhi * sizeof(E)));
+        GSN_TRY(ws.alloc(n * sizeof(E)));
+        GSN_TRY(hip_status(hipMemcpy(T.ptr, t.data(), n1 * sizeof(E), hipMemcpyHostToDevice), "hipMemcpy"));
+        GSN_TRY(hip_status(hipMemcpy(lo.ptr, vlo.data(), nlo * sizeof(E), hipMemcpyHostToDevice), "hipMemcpy"));
+        GSN_TRY(hip_status(hipMemcpy(hi.ptr, vhi.data(), nhi * sizeof(E), hipMemcpyHostToDevice), "hipMemcpy"));
+        num_passes = 2;
+        device_bytes = (n1 + nlo + nhi + n) * sizeof(E);
+        if (a > 0) {
+            // the rows: radix-2 transforms over w^n1 (a primitive 2^a-th root)
+            const H wr = w.pow(n1);
+            std::vector<gsn_batch_item> items(n1);
+            for (uint32_t k1 = 0; k1 < n1; ++k1) {
+                E *row = (E *)ws.ptr + (size_t)k1 * n2;
+                items[k1] = {FID, a, inv, wr.im_rep, row, row};
+            }
+            rows.reset(new BatchImpl());
+            GSN_TRY(build_batch(*rows, items.data(), items.size(), 0));
+            num_passes += (int)rows->groups.size() + (int)rows->copies.size();
+        }
+        return GSN_OK;
+    }
+
+    int execute(const void *in, void *out, hipStream_t st) override
+    {
+        GSN_TRY(take_injected_fault());
+        const size_t n2 = (size_t)1 << a, n = (size_t)n1 * n2;
+        hipLaunchKernelGGL((mixed_odd_kernel<O, MI::MAXN1>), dim3((unsigned)n2), dim3(MIX_THREADS), 0, st,
+                           (E *)ws.ptr, (const E *)in, n1, (uint32_t)n2, (const E *)T.ptr, (const E *)lo.ptr,
+                           (const E *)hi.ptr, h, rad);
+        GSN_TRY(hip_status(hipGetLastError(), "mixed-radix launch"));
+        if (rows)
+            GSN_TRY(rows->enqueue(st));
+        hipLaunchKernelGGL(mixed_transpose_kernel<O>, dim3(blocks_for(n)), dim3(256), 0, st, (E *)out,
+                           (const E *)ws.ptr, n1, a, n);
+        return hip_status(hipGetLastError(), "mixed-radix transpose launch");
+    }
+};
+
+template <int FID>
+int create_mixed_plan(PlanBase **out, size_t n, int inverse)
+{
+    using MI = MixInfo<FID>;
+    uint32_t a = 0;
+    size_t m = n;
+    while (m && (m & 1) == 0) {
+        m >>= 1;
+        ++a;
+    }
+    if (n == 0 || m == 1 || m > (size_t)MI::MAXN1 || MI::ODD_MAX % m != 0 || a > FieldInfo<FID>::TWO_ADICITY)
+        return set_error(GSN_ERR_INVALID_ARGUMENT,
+                         "mixed-radix size " + std::to_string(n) + " must be n1 2^a with 1 < n1 <= " +
+                             std::to_string(MI::MAXN1) + " dividing " + std::to_string(MI::ODD_MAX) +
+                             " and a <= the 2-adicity");
+    std::unique_ptr<MixedPlan<FID>> p(new MixedPlan<FID>());
+    GSN_TRY(p->init((uint32_t)m, a, inverse));
+    *out = p.release();
+    return GSN_OK;
+}
+
 template <int FID>
 int create_ext_plan(PlanBase **out, unsigned log_n, const void *shift, int inverse)
 {
@@ -507,6 +720,23 @@ int gsn_inner_product(int field, void *d_out, const void *d_a, const void *d_b, 
     case GSN_FIELD_MNT4753_FR: return dot_t<Ops753<Mnt4753FrParams>>(d_out, d_a, d_b, n, st);
     default: return dot_t<Ops753<Mnt6753FrParams>>(d_out, d_a, d_b, n, st);
     }
+}
+
+int gsn_plan_create_mixed(gsn_plan **plan, int field, size_t n, int inverse)
+{
+    if (!plan)
+        return set_error(GSN_ERR_INVALID_ARGUMENT, "plan is NULL");
+    *plan = nullptr;
+    GSN_TRY(ensure_device());
+    PlanBase *impl = nullptr;
+    switch (field) {
+    case GSN_FIELD_FP32: GSN_TRY(create_mixed_plan<GSN_FIELD_FP32>(&impl, n, inverse)); break;
+    case GSN_FIELD_MNT4753_FR: GSN_TRY(create_mixed_plan<GSN_FIELD_MNT4753_FR>(&impl, n, inverse)); break;
+    case GSN_FIELD_MNT6753_FR: GSN_TRY(create_mixed_plan<GSN_FIELD_MNT6753_FR>(&impl, n, inverse)); break;
+    default: return set_error(GSN_ERR_INVALID_ARGUMENT, "unknown field id");
+    }
+    *plan = new gsn_plan{impl};
+    return GSN_OK;
 }
 
 int gsn_plan_create_ext(gsn_plan **plan, int field, unsigned log_n, const void *shift, int inverse)

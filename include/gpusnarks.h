@@ -164,6 +164,17 @@ int gsn_plan_destroy(gsn_plan *plan);
  * radix-2 only (cuda/fft_kernel.cu:117-147); the construction is libfqfft's
  * extended_radix2_domain.  Executes with gsn_plan_execute. */
 int gsn_plan_create_ext(gsn_plan **plan, int field, unsigned log_n, const void *shift, int inverse);
+/* A plan for n = n1 * 2^a points with an odd n1 > 1 dividing the 3/5/7-smooth
+ * part of p - 1 (3 for the 32-bit field; 3 * 5^2 * 7 for MNT4-753 Fr, up to
+ * 525; 3^2 * 5^4 * 7^2 for MNT6-753 Fr, up to n1 = 735), a <= the 2-adicity:
+ * X_k = sum_j x_j w^(j k) over the primitive n-th root w = w_{2^a} w_{n1}
+ * (the standard 2^a-th root times ROOT_ODD^(ODD_MAX / n1) of
+ * gsn_field_params.h); inverse != 0: the inverse, with n^-1.  A four-step
+ * factorisation: odd DFTs of the n1-element columns in LDS (radix 3, 5, 7),
+ * radix-2 transforms of the rows, a transpose.  No counterpart in the
+ * reference (radix-2 only); it takes MNT4-753 Fr past 2^16 (up to
+ * 2^15 * 525).  Executes with gsn_plan_execute (d_out may equal d_in). */
+int gsn_plan_create_mixed(gsn_plan **plan, int field, size_t n, int inverse);
 /* Transforms d_in into d_out (device pointers; d_out may equal d_in) on
  * `stream`.  Asynchronous; the plan's workspace is reused, so calls on one
  * plan must not overlap in time. */

@@ -624,3 +624,80 @@ void or_g1_msm(const or_field *f, const uint32_t *coeff_a, uint32_t *r, const ui
     }
     memcpy(r, acc, sizeof(uint32_t) * ps);
 }
+
+/* ------------------------------------------------------------ mixed radix
+ * Transforms of n = n2 * n1 points, n2 = 2^a and n1 odd (a product of 3, 5
+ * and 7 dividing p - 1), over omega = w2 * w1 (w2 a primitive n2-th root,
+ * w1 a primitive n1-th root; their product is a primitive n-th root).  The
+ * reference has no such transform (its FFT is radix-2 only); these restate
+ * the DFT's definition and its four-step factorisation. */
+
+/* out[k] = sum_j x[j] omega^(j k), the definition, O(n^2) */
+void or_dft_direct(const or_field *f, uint32_t *out, const uint32_t *x, size_t n, const uint32_t *omega)
+{
+    const int s = f->nlimbs;
+    uint32_t wk[OR_MAX_LIMBS], wjk[OR_MAX_LIMBS], t[OR_MAX_LIMBS], acc[OR_MAX_LIMBS];
+    memcpy(wk, f->one, sizeof(uint32_t) * s); /* omega^k */
+    for (size_t k = 0; k < n; ++k) {
+        memset(acc, 0, sizeof(acc));
+        memcpy(wjk, f->one, sizeof(uint32_t) * s); /* omega^(j k) */
+        for (size_t j = 0; j < n; ++j) {
+            or_mul(f, t, EL(x, j, s), wjk);
+            or_add(f, acc, acc, t);
+            or_mul(f, wjk, wjk, wk);
+        }
+        memcpy(EL(out, k, s), acc, sizeof(uint32_t) * s);
+        or_mul(f, wk, wk, omega);
+    }
+}
+
+/* The same transform by the four-step factorisation n = n1 * n2, with
+ * j = n2 j1 + j2 and k = k1 + n1 k2:
+ *   Y[k1][j2] = omega^(j2 k1) sum_j1 x[n2 j1 + j2] w1'^(j1 k1),  w1' = omega^n2
+ *   X[k1 + n1 k2] = sum_j2 Y[k1][j2] w2'^(j2 k2),                w2' = omega^n1
+ * (odd DFTs directly, the rows by the radix-2 FFT).  inverse != 0: the
+ * inverse transform (omega^-1, then n^-1). */
+void or_mixed_fft(const or_field *f, uint32_t *a, size_t n1, size_t n2, const uint32_t *w2, const uint32_t *w1,
+                  int inverse)
+{
+    const int s = f->nlimbs;
+    const size_t n = n1 * n2;
+    uint32_t om[OR_MAX_LIMBS], w1p[OR_MAX_LIMBS], w2p[OR_MAX_LIMBS], t[OR_MAX_LIMBS], acc[OR_MAX_LIMBS];
+    or_mul(f, om, w2, w1);
+    if (inverse)
+        or_inverse(f, om, om);
+    or_pow(f, w1p, om, (uint64_t)n2);
+    or_pow(f, w2p, om, (uint64_t)n1);
+    uint32_t *Y = (uint32_t *)malloc(sizeof(uint32_t) * s * n);
+    uint32_t *col = (uint32_t *)malloc(sizeof(uint32_t) * s * n1);
+    uint32_t *dft = (uint32_t *)malloc(sizeof(uint32_t) * s * n1);
+    for (size_t j2 = 0; j2 < n2; ++j2) {
+        for (size_t j1 = 0; j1 < n1; ++j1)
+            memcpy(EL(col, j1, s), EL(a, n2 * j1 + j2, s), sizeof(uint32_t) * s);
+        or_dft_direct(f, dft, col, n1, w1p);
+        uint32_t tw[OR_MAX_LIMBS], step[OR_MAX_LIMBS];
+        or_pow(f, step, om, (uint64_t)j2); /* omega^j2 */
+        memcpy(tw, f->one, sizeof(uint32_t) * s);
+        for (size_t k1 = 0; k1 < n1; ++k1) {
+            or_mul(f, EL(Y, k1 * n2 + j2, s), EL(dft, k1, s), tw);
+            or_mul(f, tw, tw, step);
+        }
+    }
+    for (size_t k1 = 0; k1 < n1; ++k1)
+        or_serial_radix2_fft(f, EL(Y, k1 * n2, s), n2, w2p, f->one, 0);
+    uint32_t ninv[OR_MAX_LIMBS], nraw[OR_MAX_LIMBS] = {0}, nm[OR_MAX_LIMBS];
+    nraw[0] = (uint32_t)n;
+    or_to_mont(f, nm, nraw);
+    or_inverse(f, ninv, nm);
+    for (size_t k1 = 0; k1 < n1; ++k1)
+        for (size_t k2 = 0; k2 < n2; ++k2) {
+            memcpy(t, EL(Y, k1 * n2 + k2, s), sizeof(uint32_t) * s);
+            if (inverse)
+                or_mul(f, t, t, ninv);
+            memcpy(EL(a, k1 + n1 * k2, s), t, sizeof(uint32_t) * s);
+        }
+    (void)acc;
+    free(Y);
+    free(col);
+    free(dft);
+}

@@ -48,6 +48,9 @@ class Oracle:
         L.or_parallel_phase1_sample.argtypes = [ctypes.POINTER(OrField), _u32p, ctypes.c_size_t, _u32p,
                                                 ctypes.c_uint, ctypes.c_size_t, ctypes.c_size_t, _u32p, ctypes.c_int]
         L.or_inverse_fft.argtypes = [ctypes.POINTER(OrField), _u32p, ctypes.c_size_t, _u32p]
+        L.or_dft_direct.argtypes = [ctypes.POINTER(OrField), _u32p, _u32p, ctypes.c_size_t, _u32p]
+        L.or_mixed_fft.argtypes = [ctypes.POINTER(OrField), _u32p, ctypes.c_size_t, ctypes.c_size_t, _u32p, _u32p,
+                                   ctypes.c_int]
         L.or_g1_add.argtypes = [ctypes.POINTER(OrField), _u32p, _u32p, _u32p, _u32p]
         L.or_g1_dbl.argtypes = [ctypes.POINTER(OrField), _u32p, _u32p, _u32p]
         L.or_g1_mul.argtypes = [ctypes.POINTER(OrField), _u32p, _u32p, _u32p, _u32p, ctypes.c_int]
@@ -109,6 +112,18 @@ class Oracle:
         self.lib.or_parallel_radix2_fft_inner(self.field(fid), _p(a), a.shape[0],
                                               _p(np.ascontiguousarray(omega, np.uint32)), log_cpus,
                                               _p(one), int(literal), threads)
+        return a
+
+    def dft_direct(self, fid, x, omega):
+        a = np.ascontiguousarray(x, np.uint32)
+        out = np.zeros_like(a)
+        self.lib.or_dft_direct(self.field(fid), _p(out), _p(a), a.shape[0], _p(np.ascontiguousarray(omega, np.uint32)))
+        return out
+
+    def mixed_fft(self, fid, x, n1, w2, w1, inverse=False):
+        a = np.ascontiguousarray(x, np.uint32).copy()
+        self.lib.or_mixed_fft(self.field(fid), _p(a), n1, a.shape[0] // n1, _p(np.ascontiguousarray(w2, np.uint32)),
+                              _p(np.ascontiguousarray(w1, np.uint32)), int(inverse))
         return a
 
     def extended_fft(self, fid, x, omega_half, shift, inverse=False):
@@ -191,3 +206,24 @@ class RefHarness:
         out = np.zeros(24, np.uint32)
         self.lib.ref_scalar_modulus(_p(out))
         return out
+
+
+def odd_root(fid):
+    """(M, w): the 3/5/7-smooth part M of p - 1 and the primitive M-th root of
+    unity w (a plain integer) that include/gsn_field_params.h holds as
+    ODD_MAX / ROOT_ODD."""
+    import re
+    text = open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "include",
+                             "gsn_field_params.h")).read()
+    struct = {0: "Fp32Params", 1: "Mnt4753FrParams", 2: "Mnt6753FrParams"}[fid]
+    body = text[text.index("struct %s {" % struct):]
+    body = body[:body.index("\n};")]
+    M = int(re.search(r"ODD_MAX = (\d+)u", body).group(1))
+    m = re.search(r"ROOT_ODD(?:\[24\])? = \{?([^;}]*)\}?;", body).group(1)
+    vals = [int(v.strip().rstrip("u"), 16) for v in m.split(",")]
+    L = NLIMBS[fid]
+    mod = re.search(r"MOD\[24\] = \{([^}]*)\}", body) if L > 1 else None
+    p = (sum(int(v.strip().rstrip("u"), 16) << (32 * i) for i, v in enumerate(mod.group(1).split(",")))
+         if mod else int(re.search(r"P = (0x[0-9a-f]+)u", body).group(1), 16))
+    rinv = pow(1 << (32 * L), -1, p)
+    return M, sum(v << (32 * i) for i, v in enumerate(vals)) * rinv % p

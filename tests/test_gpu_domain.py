@@ -251,3 +251,76 @@ def test_inner_product_reference_harness_shape():
     g.inner_product(field, dout.ptr, da.ptr, da.ptr, n)
     g.synchronize()
     assert np.array_equal(dout.download((1, L))[0], mont(field, n * 1234 * 1234 % p))
+
+
+def cheap_residues(field, n, seed):
+    """Uniform-looking canonical elements without rejection sampling: random
+    limbs below p's top limb (for the large sizes)."""
+    rng = np.random.default_rng(seed)
+    L = g.NLIMBS[field]
+    x = rng.integers(0, 1 << 32, size=(n, L), dtype=np.uint64).astype(np.uint32)
+    top = g.MODULI[field] >> (32 * (L - 1))
+    x[:, L - 1] = (x[:, L - 1].astype(np.uint64) % top).astype(np.uint32)
+    return x
+
+
+def mixed_roots(field, n1, a):
+    from tests.oracle_lib import odd_root
+    M, w = odd_root(field)
+    p = g.MODULI[field]
+    return g.root_of_unity(field, a), mont(field, pow(w, M // n1, p))
+
+
+@pytest.mark.parametrize("field,n1,a", [(g.FIELD_FP32, 3, 10), (g.FIELD_MNT4753_FR, 105, 6),
+                                        (g.FIELD_MNT4753_FR, 525, 2), (g.FIELD_MNT4753_FR, 15, 9),
+                                        (g.FIELD_MNT4753_FR, 7, 0), (g.FIELD_MNT6753_FR, 245, 3),
+                                        (g.FIELD_MNT6753_FR, 9, 5)])
+def test_mixed_radix_matches_oracle(orc, field, n1, a):
+    """Plan.mixed (n = n1 2^a over w = w_{2^a} ROOT_ODD^(ODD_MAX/n1)) against
+    the oracle's four-step restatement (itself pinned to the O(n^2) DFT),
+    forward and inverse, bit-exact."""
+    n = n1 << a
+    x = residues(field, n, 0x3A0 + n)
+    w2, w1 = mixed_roots(field, n1, a)
+    say("mixed radix field", field, "n1", n1, "a", a)
+    want = orc.mixed_fft(field, x, n1, w2, w1)
+    fwd, inv = g.Plan.mixed(field, n), g.Plan.mixed(field, n, inverse=True)
+    d, e = g.DeviceArray.from_numpy(x), g.DeviceArray(x.nbytes)
+    fwd.execute(d.ptr, e.ptr)
+    g.synchronize()
+    assert np.array_equal(e.download(x.shape), want)
+    inv.execute(e.ptr, e.ptr)  # in place
+    g.synchronize()
+    assert np.array_equal(e.download(x.shape), x)
+
+
+def test_mixed_radix_mnt4_largest():
+    """MNT4-753 Fr at n = 2^15 * 525 (17 million points, past its 2-adicity
+    by a factor 1050): the inverse undoes the forward, and X_0 = sum x."""
+    field, n = g.FIELD_MNT4753_FR, 525 << 15
+    p = g.MODULI[field]
+    x = cheap_residues(field, n, 0x525)
+    say("mixed radix MNT4 n", n)
+    fwd, inv = g.Plan.mixed(field, n), g.Plan.mixed(field, n, inverse=True)
+    d = g.DeviceArray.from_numpy(x)
+    fwd.execute(d.ptr, d.ptr)
+    g.synchronize()
+    X0 = sum(int(v) << (32 * i) for i, v in enumerate(d.download((1, 24))[0]))
+    s = 0
+    for i in range(24):  # sum of the representations mod p, limb column by column
+        s += int(x[:, i].astype(np.uint64).sum()) << (32 * i)
+    assert X0 == s % p
+    inv.execute(d.ptr, d.ptr)
+    g.synchronize()
+    assert np.array_equal(d.download(x.shape), x)
+
+
+def test_mixed_radix_errors():
+    with pytest.raises(g.GsnError):
+        g.Plan.mixed(g.FIELD_MNT4753_FR, 1 << 10)  # no odd factor
+    with pytest.raises(g.GsnError):
+        g.Plan.mixed(g.FIELD_MNT4753_FR, 11 * 4)  # 11 is not a supported factor
+    with pytest.raises(g.GsnError):
+        g.Plan.mixed(g.FIELD_MNT4753_FR, 1575 * 2)  # 3^2 does not divide p - 1 of MNT4-753 Fr
+    with pytest.raises(g.GsnError):
+        g.Plan.mixed(g.FIELD_MNT4753_FR, 3 << 16)  # 2^16 exceeds the 2-adicity

@@ -330,3 +330,60 @@ def test_g1_msm_matches_affine(orc, name, fid, a, sfid):
     pts = np.stack([aff.to_projective(P, q, R) for P in Ps])
     got = orc.g1_msm(fid, a_m, pts, np.stack([synth.from_int(k, 24) for k in ks]))
     assert aff.to_affine(got, q, R) == want
+
+
+# ---------------------------------------------------------------- mixed radix
+def _odd_root(fid):
+    from tests.oracle_lib import odd_root
+    return odd_root(fid)
+
+
+@pytest.mark.parametrize("fid", [0, 1, 2])
+def test_odd_roots_of_unity(fid):
+    """ROOT_ODD has order exactly ODD_MAX (the 3/5/7-smooth part of p - 1)."""
+    orc = Oracle()
+    p = synth.to_int(orc.params(fid)["mod"])
+    M, w = _odd_root(fid)
+    assert (p - 1) % M == 0 and pow(w, M, p) == 1
+    for l in (3, 5, 7):
+        if M % l == 0:
+            assert pow(w, M // l, p) != 1
+    assert M == {0: 3, 1: 3 * 25 * 7, 2: 9 * 625 * 49}[fid]
+
+
+@pytest.mark.parametrize("fid,n", [(0, 3), (0, 12), (1, 15), (1, 21), (2, 9), (2, 25)])
+def test_dft_direct_matches_definition(orc, fid, n):
+    L = NLIMBS[fid]
+    p = synth.to_int(orc.params(fid)["mod"])
+    R = 1 << (32 * L)
+    M, wo = _odd_root(fid)
+    m = n
+    while m % 2 == 0:
+        m //= 2
+    n2 = n // m
+    w2 = pow({0: 5, 1: 13, 2: 11}[fid], (p - 1) // n2, p)
+    w = w2 * pow(wo, M // m, p) % p
+    x = synth.uniform_residues(0xD0 + n, n, p, L)
+    xv = [synth.to_int(r) * pow(R, -1, p) % p for r in x]
+    out = orc.dft_direct(fid, x, synth.from_int(w * R % p, L))
+    want = [sum(xv[j] * pow(w, j * k, p) for j in range(n)) % p for k in range(n)]
+    assert [synth.to_int(r) * pow(R, -1, p) % p for r in out] == want
+
+
+@pytest.mark.parametrize("fid,n1,n2", [(0, 3, 16), (1, 15, 8), (1, 105, 4), (1, 525, 2), (2, 9, 8), (2, 245, 4),
+                                       (1, 7, 1)])
+def test_mixed_fft_matches_direct_dft(orc, fid, n1, n2):
+    """The four-step restatement equals the O(n^2) definition, and its
+    inverse undoes it."""
+    L = NLIMBS[fid]
+    p = synth.to_int(orc.params(fid)["mod"])
+    R = 1 << (32 * L)
+    M, wo = _odd_root(fid)
+    w1 = pow(wo, M // n1, p)
+    w2 = pow({0: 5, 1: 13, 2: 11}[fid], (p - 1) // n2, p)
+    mont = lambda v: synth.from_int(v * R % p, L)
+    n = n1 * n2
+    x = synth.uniform_residues(0xE0 + n, n, p, L)
+    got = orc.mixed_fft(fid, x, n1, mont(w2), mont(w1))
+    assert np.array_equal(got, orc.dft_direct(fid, x, mont(w1 * w2 % p)))
+    assert np.array_equal(orc.mixed_fft(fid, got, n1, mont(w2), mont(w1), inverse=True), x)
