@@ -309,6 +309,20 @@ __global__ __launch_bounds__(1 << (LOG_R + LOG_W - LOG_E)) void ntt_pass_multi_k
 // One tile of R = 2^log_r rows by W = 2^log_w columns (log_r + log_w <=
 // LOG_TILE) with radix-2 steps.  Threads past R*W/2 only join the barriers.
 //
+// In-place decimation in time over positions i = 0..R-1 of each column:
+// thread t = jj W + c starts with positions 2 jj and 2 jj + 1, i.e. input
+// rows brev(jj) and brev(jj) + R/2 (brev over log_r - 1 bits), and step s
+// pairs positions i and i + 2^s.  Between steps s - 1 and s, position bit s
+// (held by thread bit tau = log_w + s - 1) and the slot bit trade places:
+// threads t and t ^ 2^tau exchange through LDS, each writing both outputs
+// into the slots their next owners read.  For tau < 6 the partner is in the
+// same wavefront, whose LDS operations execute in order, so the exchange
+// needs no workgroup barrier (only a compiler fence); a partner in another
+// wavefront costs a barrier before the writes and one after.  After the
+// last step thread t holds positions jj and jj + R/2, which are output rows
+// jj and jj + R/2 (the decimation in time reads bit-reversed and writes
+// natural order).
+//
 // The pass twiddle w_{Ns R}^(r k) is not applied to the inputs: folded into
 // the steps, step s multiplies x1 by w_{Ns 2^(s+1)}^(k + Ns kk), kk = jj mod
 // 2^s (for Ns = 1, k = 0 these are the plain radix-2 twiddles), so a pass of
@@ -359,9 +373,11 @@ __device__ __forceinline__ void tile753_body(const PassArgs<F> &a, int log_r, in
     elem x[2];
     tw w;  // twiddle of the current step
     if (on) {
-        // step 0: rows jj and jj + R/2 from global memory
-        x[1] = F::load(a.in, j + (size_t)(jj + half) * nR);
-        x[0] = F::load(a.in, j + (size_t)jj * nR);
+        // step 0: positions 2 jj and 2 jj + 1 = rows brev(jj) and
+        // brev(jj) + R/2 from global memory
+        const int jr = log_r > 1 ? (int)(__builtin_bitreverse32((uint32_t)jj) >> (33 - log_r)) : 0;
+        x[1] = F::load(a.in, j + (size_t)(jr + half) * nR);
+        x[0] = F::load(a.in, j + (size_t)jr * nR);
         if (log_ns != 0)
             x[1] = F::mul(x[1], F::load_tw_planes(tab, planes, k));
         w = step_tw(1, x[1].v[23]);
@@ -370,24 +386,34 @@ __device__ __forceinline__ void tile753_body(const PassArgs<F> &a, int log_r, in
         x[0] = s0;
     }
     for (int s = 1; s < log_r; ++s) {
-        // the previous step (stride 2^(s-1)) wrote column jj to rows
-        // (jj >> (s-1)) * 2^s + kk' + e * 2^(s-1), kk' = jj mod 2^(s-1)
-        const int ps = s - 1;
+        // x[e] is position bit s - 1 = e; thread bit tau holds position bit
+        // s.  Output e goes to the thread with bit tau = e, into its slot
+        // (own bit tau): LDS element (own bit tau) * R W/2 + owner.
+        const int tau = log_w + s - 1;
+        const bool cross = tau >= 6;  // the partner is in another wavefront
+        if (cross)
+            __syncthreads();  // every wavefront has read the previous exchange
         if (on) {
-            const int row = ((jj >> ps) << s) + (jj & ((1 << ps) - 1));
-            F::lds_store(lds, STRIDE, row * W + c, x[0]);
-            F::lds_store(lds, STRIDE, (row + (1 << ps)) * W + c, x[1]);
+            const int m = 1 << tau;
+            const int slot = (t & m) ? active : 0;
+            F::lds_store(lds, STRIDE, slot + (t & ~m), x[0]);
+            F::lds_store(lds, STRIDE, slot + (t | m), x[1]);
         }
-        __syncthreads();
+        if (cross) {
+            __syncthreads();
+        } else {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
         if (on) {
-            x[1] = F::mul(F::lds_load(lds, STRIDE, (jj + half) * W + c), w);
+            x[1] = F::mul(F::lds_load(lds, STRIDE, active + t), w);
             w = step_tw(s + 1, x[1].v[23]);
-            x[0] = F::lds_load(lds, STRIDE, jj * W + c);
+            x[0] = F::lds_load(lds, STRIDE, t);
             const elem s0 = F::add(x[0], x[1]);
             x[1] = F::sub(x[0], x[1]);
             x[0] = s0;
         }
-        __syncthreads();
     }
     // x[e] holds output row jj + e*R/2 of column c
     if (on) {
@@ -404,8 +430,9 @@ __device__ __forceinline__ void tile753_body(const PassArgs<F> &a, int log_r, in
         // first pass: the tile's R*W outputs are one contiguous run
         // (out[j*R + r]); stage it in LDS and write it as consecutive
         // 16-byte quads, lane after lane, with non-temporal stores (the next
-        // pass reads it from HBM, not from this CU's caches).  The step
-        // loop ended on a barrier, so LDS is free.
+        // pass reads it from HBM, not from this CU's caches).  The barrier
+        // frees LDS: every wavefront has read its last exchange.
+        __syncthreads();
         if (on) {
             F::lds_store(lds, STRIDE, jj * W + c, x[0]);
             F::lds_store(lds, STRIDE, (jj + half) * W + c, x[1]);
@@ -433,6 +460,7 @@ __device__ __forceinline__ void tile753_body(const PassArgs<F> &a, int log_r, in
         }
     } else {
         // one contiguous run of R*W outputs: transpose through LDS
+        __syncthreads();
         if (on) {
             F::lds_store(lds, STRIDE, jj * W + c, x[0]);
             F::lds_store(lds, STRIDE, (jj + half) * W + c, x[1]);

@@ -1,10 +1,13 @@
 """The kernels' index maps and twiddle schedules, restated in Python over a
 small prime field and checked against the O(n^2) DFT (CPU only).
 
-* 753-bit tiles (gpusnarks_amd/csrc/ntt_kernels.h, tile753_body): radix-2
-  Stockham steps inside an R x W tile with the pass twiddle folded into the
-  steps -- step s multiplies x1 by w_{Ns 2^(s+1)}^(k + Ns kk), kk = jj mod 2^s
-  -- and the per-pass step-table layout of build_steps753_kernel.
+* 753-bit tiles (gpusnarks_amd/csrc/ntt_kernels.h, tile753_body): in-place
+  radix-2 decimation in time inside an R x W tile (bit-reversed rows in,
+  natural rows out) with the pass twiddle folded into the steps -- step s
+  multiplies x1 by w_{Ns 2^(s+1)}^(k + Ns kk), kk = jj mod 2^s -- the
+  exchange between steps (threads t and t ^ 2^tau trade through LDS slots,
+  within one wavefront when tau < 6), and the per-pass step-table layout of
+  build_steps753_kernel.
 * 32-bit tiles (pass32_body): radix-E register steps with a first step of
   radix E0 and pass twiddles formed as A B^g C^e.
 """
@@ -35,39 +38,58 @@ def step_table(log_n, log_ns, log_r, w):
     return tab
 
 
+def brev(x, bits):
+    return int(format(x, "0%db" % bits)[::-1], 2) if bits else 0
+
+
 def folded_radix2(a, w, passes, log_w):
     n = len(a)
     log_n = n.bit_length() - 1
     cur, log_ns = list(a), 0
     for log_r in passes:
         R, Ns, nR = 1 << log_r, 1 << log_ns, n >> log_r
-        W = min(1 << log_w, nR)
+        lw = min(log_w, log_n - log_r)
+        W = 1 << lw
+        active = R * W // 2  # threads t = jj W + c
         tab = step_table(log_n, log_ns, log_r, w)
         out = [None] * n
         for blk in range(nR // W):
             j0 = blk * W
-            tile = [[cur[j0 + c + r * nR] for c in range(W)] for r in range(R)]
+            x = []
+            for t in range(active):
+                c, jr = t & (W - 1), brev(t >> lw, log_r - 1)
+                x.append([cur[j0 + c + jr * nR], cur[j0 + c + (jr + R // 2) * nR]])
             for s in range(log_r):
-                ns2 = 1 << s
-                new = [[None] * W for _ in range(R)]
-                for c in range(W):
+                if s:
+                    # output e of thread t -> LDS element (t & m ? active : 0) + owner
+                    tau = lw + s - 1
+                    m = 1 << tau
+                    lds = [None] * (2 * active)
+                    for t in range(active):
+                        slot = active if t & m else 0
+                        # the partner is in the same 64-lane wavefront unless tau >= 6
+                        assert ((t ^ m) >> 6 == t >> 6) == (tau < 6)
+                        for e, owner in enumerate((t & ~m, t | m)):
+                            assert lds[slot + owner] is None
+                            lds[slot + owner] = x[t][e]
+                    x = [[lds[t], lds[active + t]] for t in range(active)]
+                for t in range(active):
+                    c, jj = t & (W - 1), t >> lw
                     k = (j0 + c) & (Ns - 1)
-                    for jj in range(R // 2):
-                        kk = jj & (ns2 - 1)
-                        if s == 0:
-                            t = tab[k] if log_ns else 1          # record k of step 0
-                        else:
-                            t = tab[(((1 << s) - 1) << log_ns) + (k << s) + kk]
-                        x0, x1 = tile[jj][c], tile[jj + R // 2][c] * t % P
-                        row = (jj // ns2) * ns2 * 2 + kk
-                        new[row][c] = (x0 + x1) % P
-                        new[row + ns2][c] = (x0 - x1) % P
-                tile = new
-            for r in range(R):
-                for c in range(W):
-                    j = j0 + c
-                    k = j & (Ns - 1)
-                    out[(j // Ns) * Ns * R + k + r * Ns] = tile[r][c]
+                    kk = jj & ((1 << s) - 1)
+                    if s == 0:
+                        tw = tab[k] if log_ns else 1          # record k of step 0
+                    else:
+                        tw = tab[(((1 << s) - 1) << log_ns) + (k << s) + kk]
+                    x0, x1 = x[t][0], x[t][1] * tw % P
+                    x[t] = [(x0 + x1) % P, (x0 - x1) % P]
+            # thread t holds output rows jj and jj + R/2 of column c
+            for t in range(active):
+                c, jj = t & (W - 1), t >> lw
+                j = j0 + c
+                k = j & (Ns - 1)
+                for e in range(2):
+                    out[(j // Ns) * Ns * R + k + (jj + e * (R // 2)) * Ns] = x[t][e]
         cur, log_ns = out, log_ns + log_r
     return cur
 
@@ -121,7 +143,8 @@ def radix_e(a, w, passes, log_w, log_e):
 
 
 @pytest.mark.parametrize("log_n,passes,log_w", [(4, [4], 0), (6, [3, 3], 1), (7, [3, 4], 2), (8, [4, 4], 0),
-                                                (8, [2, 3, 3], 1), (9, [5, 4], 3), (9, [3, 3, 3], 0)])
+                                                (8, [2, 3, 3], 1), (9, [5, 4], 3), (9, [3, 3, 3], 0),
+                                                (1, [1], 0), (10, [9, 1], 0), (10, [6, 4], 2)])
 def test_folded_radix2_schedule(log_n, passes, log_w):
     rng = random.Random(log_n * 100 + len(passes))
     n = 1 << log_n
