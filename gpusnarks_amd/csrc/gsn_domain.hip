@@ -200,62 +200,106 @@ __global__ __launch_bounds__(256) void ext_inv_kernel(typename F::elem_t *a, con
 
 // Mixed-radix transforms, n = n1 * n2 with n1 odd (3, 5, 7 factors of p - 1)
 // and n2 = 2^a, by the four-step factorisation (j = n2 j1 + j2, k = k1 + n1 k2):
-//   Y[k1][j2] = w^(j2 k1) sum_j1 x[n2 j1 + j2] w^(n2 j1 k1)   (mixed_odd_kernel)
+//   Y[k1][j2] = w^(j2 k1) sum_j1 x[n2 j1 + j2] w^(n2 j1 k1)   (mixed_pass_kernel)
 //   Z[k1][k2] = sum_j2 Y[k1][j2] w^(n1 j2 k2)                  (radix-2 plans on rows)
 //   X[k1 + n1 k2] = Z[k1][k2]                                   (mixed_transpose_kernel)
-// The odd DFT of one column runs in LDS as radix-3/5/7 Stockham passes: a
-// pass of radix q over stride L twiddles input r of group j by
-// w_{Lq}^(r (j mod L)) and writes output s to (j / L) L q + (j mod L) + s L.
-// Products are general Montgomery products against tables of Montgomery
-// forms: T[e] = (w^n2)^e for e < n1, and w^e = lo[e mod 2^h] hi[e >> h].
-constexpr int MIX_THREADS = 128;
+// The odd DFT of every column runs as radix-3/5/7 Stockham passes, one
+// kernel per pass over all columns (thread = column j2 x group j, adjacent
+// threads on adjacent columns, so every access is coalesced): a pass of
+// radix q over stride L twiddles input r of group j by w_{Lq}^(r (j mod L))
+// and writes output s to position (j / L) L q + (j mod L) + s L.  Position
+// i of column j2 lives at i n2 + j2 in every buffer.  The last pass also
+// multiplies by w^(j2 k1) and so leaves Y row-major by k1.  Products are
+// general Montgomery products against tables of Montgomery forms:
+// T[e] = (w^n2)^e for e < n1, and w^e = lo[e mod 2^h] hi[e >> h].
+//
+// Each q-point DFT folds its inputs in pairs: with u = w_q, a_m = x_m +
+// x_{q-m} and b_m = x_m - x_{q-m} (m = 1..(q-1)/2),
+//   X_0 = x_0 + sum a_m,  X_k, X_{q-k} = x_0 + U_k +- V_k,
+//   U_k = sum_m a_m C[m k mod q],  V_k = sum_m b_m S[m k mod q],
+// C[e] = (u^e + u^-e)/2, S[e] = (u^e - u^-e)/2 (the table CS, per radix):
+// (q-1)^2/2 products instead of (q-1)^2.
 struct MixRadices {
     uint32_t count;
-    uint32_t q[16];
+    uint32_t q[24];
 };
+// CS layout: radix 3 at 0, 5 at 6, 7 at 16; C[e] at off + e, S[e] at off + q + e
+__host__ __device__ constexpr uint32_t mix_cs_offset(uint32_t q) { return q == 3 ? 0u : (q == 5 ? 6u : 16u); }
+constexpr uint32_t MIX_CS_ELEMS = 30;
 
-template <class O, int MAXN1>
-__global__ __launch_bounds__(MIX_THREADS) void mixed_odd_kernel(typename O::elem *ws, const typename O::elem *in,
-                                                                uint32_t n1, uint32_t n2, const typename O::elem *T,
-                                                                const typename O::elem *lo,
-                                                                const typename O::elem *hi, uint32_t h,
-                                                                MixRadices rad)
+template <class O, int Q, bool LAST>
+__global__ __launch_bounds__(256) void mixed_pass_kernel(typename O::elem *dst, const typename O::elem *src,
+                                                         uint32_t n1, uint32_t log_n2, uint32_t L,
+                                                         const typename O::elem *T, const typename O::elem *CS,
+                                                         const typename O::elem *lo, const typename O::elem *hi,
+                                                         uint32_t h)
 {
     using E = typename O::elem;
-    __shared__ E buf[2 * MAXN1];
-    E *A = buf, *B = buf + MAXN1;
-    const uint32_t j2 = blockIdx.x;
-    for (uint32_t j1 = threadIdx.x; j1 < n1; j1 += MIX_THREADS)
-        A[j1] = O::load(in, (size_t)n2 * j1 + j2);
-    __syncthreads();
-    uint32_t L = 1;
-    for (uint32_t pi = 0; pi < rad.count; ++pi) {
-        const uint32_t q = rad.q[pi], nq = n1 / q, span = n1 / (L * q);
-        for (uint32_t j = threadIdx.x; j < nq; j += MIX_THREADS) {
-            const uint32_t k = j % L, jb = j / L;
-            if (k != 0)  // r k span < n1: w_{Lq}^(r k) = T[r k span]
-                for (uint32_t r = 1; r < q; ++r)
-                    A[j + r * nq] = O::mul(A[j + r * nq], O::load(T, (size_t)r * k * span));
-            for (uint32_t sx = 0; sx < q; ++sx) {
-                E acc = A[j];
-                for (uint32_t r = 1; r < q; ++r)
-                    acc = O::add(acc, sx == 0 ? A[j + r * nq] : O::mul(A[j + r * nq], O::load(T, (size_t)((r * sx) % q) * nq)));
-                B[jb * L * q + k + sx * L] = acc;
-            }
+    constexpr int H = (Q - 1) / 2;
+    const uint32_t nq = n1 / Q, span = n1 / (L * Q);
+    const size_t tid = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (tid >= ((size_t)nq << log_n2))
+        return;
+    const size_t j2 = tid & (((size_t)1 << log_n2) - 1);
+    const uint32_t j = (uint32_t)(tid >> log_n2);
+    const uint32_t k = j % L, jb = j / L;
+    const E *C = CS + mix_cs_offset(Q), *S = C + Q;
+    E x[Q];
+#pragma unroll
+    for (int r = 0; r < Q; ++r)
+        x[r] = O::load(src, ((size_t)(j + r * nq) << log_n2) + j2);
+    if (k != 0)  // r k span < n1: w_{Lq}^(r k) = T[r k span]
+#pragma unroll
+        for (int r = 1; r < Q; ++r)
+            x[r] = O::mul(x[r], O::load(T, (size_t)r * k * span));
+    const uint32_t pos0 = jb * L * Q + k;
+    auto put = [&](uint32_t s, const E &v) {
+        const uint32_t pos = pos0 + s * L;
+        if (LAST) {
+            const uint64_t e = (uint64_t)j2 * pos;
+            O::store(dst, ((size_t)pos << log_n2) + j2,
+                     O::mul(O::mul(v, O::load(lo, (size_t)(e & ((1ull << h) - 1)))), O::load(hi, (size_t)(e >> h))));
+        } else {
+            O::store(dst, ((size_t)pos << log_n2) + j2, v);
         }
-        __syncthreads();
-        E *t = A;
-        A = B;
-        B = t;
-        L *= q;
+    };
+    E s0 = x[0];
+#pragma unroll
+    for (int m = 1; m <= H; ++m) {
+        const E am = O::add(x[m], x[Q - m]);
+        x[Q - m] = O::sub(x[m], x[Q - m]);  // b_m
+        x[m] = am;
+        s0 = O::add(s0, am);
     }
-    // A holds the column's DFT; twiddle by w^(j2 k1) (j2 k1 < n) and store
-    // row-major by k1
-    for (uint32_t k1 = threadIdx.x; k1 < n1; k1 += MIX_THREADS) {
-        const uint64_t e = (uint64_t)j2 * k1;
-        const E v = O::mul(O::mul(A[k1], O::load(lo, (size_t)(e & ((1ull << h) - 1)))), O::load(hi, (size_t)(e >> h)));
-        O::store(ws, (size_t)k1 * n2 + j2, v);
+    put(0, s0);
+#pragma unroll
+    for (int kk = 1; kk <= H; ++kk) {
+        E U = O::mul(x[1], O::load(C, (uint32_t)kk % Q));
+        E V = O::mul(x[Q - 1], O::load(S, (uint32_t)kk % Q));
+#pragma unroll
+        for (int m = 2; m <= H; ++m) {
+            U = O::add(U, O::mul(x[m], O::load(C, (uint32_t)(m * kk) % Q)));
+            V = O::add(V, O::mul(x[Q - m], O::load(S, (uint32_t)(m * kk) % Q)));
+        }
+        const E base = O::add(x[0], U);
+        put((uint32_t)kk, O::add(base, V));
+        put((uint32_t)(Q - kk), O::sub(base, V));
     }
+}
+
+template <class O, int Q>
+void launch_mixed_pass(bool last, typename O::elem *dst, const typename O::elem *src, uint32_t n1, uint32_t log_n2,
+                       uint32_t L, const typename O::elem *T, const typename O::elem *CS,
+                       const typename O::elem *lo, const typename O::elem *hi, uint32_t h, hipStream_t st)
+{
+    const size_t threads = ((size_t)(n1 / Q)) << log_n2;
+    const dim3 grid((unsigned)((threads + 255) / 256)), block(256);
+    if (last)
+        hipLaunchKernelGGL((mixed_pass_kernel<O, Q, true>), grid, block, 0, st, dst, src, n1, log_n2, L, T, CS, lo,
+                           hi, h);
+    else
+        hipLaunchKernelGGL((mixed_pass_kernel<O, Q, false>), grid, block, 0, st, dst, src, n1, log_n2, L, T, CS,
+                           lo, hi, h);
 }
 
 // out[k1 + n1 k2] = ws[k1 n2 + k2]
@@ -504,7 +548,7 @@ struct MixInfo<GSN_FIELD_MNT4753_FR> {
 template <>
 struct MixInfo<GSN_FIELD_MNT6753_FR> {
     using O = Ops753<Mnt6753FrParams>;
-    static constexpr int MAXN1 = 735;  // two LDS buffers of 96-byte elements within 160 KB
+    static constexpr int MAXN1 = 735;  // 3 5 7^2: the largest odd part the tests cover
     static constexpr uint32_t ODD_MAX = Mnt6753FrParams::ODD_MAX;
     static fields::ScalarMNT6 root_odd() { return fields::ScalarMNT6::from_raw(Mnt6753FrParams::ROOT_ODD); }
 };
@@ -522,7 +566,7 @@ struct MixedPlan : PlanBase {
     using E = typename O::elem;
     uint32_t n1 = 0, a = 0, h = 0;
     MixRadices rad{};
-    DeviceBuffer T, lo, hi, ws;
+    DeviceBuffer T, lo, hi, ws, ws2, CS;
     std::unique_ptr<BatchImpl> rows;
 
     static void put(std::vector<E> &v, size_t i, const H &x)
@@ -551,6 +595,20 @@ struct MixedPlan : PlanBase {
         H acc = H::one();
         for (uint32_t e = 0; e < n1; ++e, acc = acc * w1)
             put(t, e, acc);
+        // symmetric DFT constants per radix q: u = w1^(n1/q), C[e] = (u^e +
+        // u^-e)/2, S[e] = (u^e - u^-e)/2
+        std::vector<E> cs(MIX_CS_ELEMS);
+        const H half = H(2u).mul_inv();
+        for (uint32_t q : {3u, 5u, 7u}) {
+            if (n1 % q)
+                continue;
+            const H u = w1.pow(n1 / q), ui = u.mul_inv();
+            H up = H::one(), un = H::one();
+            for (uint32_t e = 0; e < q; ++e, up = up * u, un = un * ui) {
+                put(cs, mix_cs_offset(q) + e, (up + un) * half);
+                put(cs, mix_cs_offset(q) + q + e, (up - un) * half);
+            }
+        }
         // base^e = lo[e mod 2^h] hi[e >> h] (hi scaled by n1^-1 for the inverse)
         while (((size_t)1 << (2 * h)) < n)
             ++h;
@@ -566,11 +624,16 @@ struct MixedPlan : PlanBase {
         GSN_TRY(lo.alloc(nlo * sizeof(E)));
         GSN_TRY(hi.alloc(nhi * sizeof(E)));
         GSN_TRY(ws.alloc(n * sizeof(E)));
+        if (rad.count > 1)
+            GSN_TRY(ws2.alloc(n * sizeof(E)));
+        GSN_TRY(CS.alloc(MIX_CS_ELEMS * sizeof(E)));
+        GSN_TRY(hip_status(hipMemcpy(CS.ptr, cs.data(), MIX_CS_ELEMS * sizeof(E), hipMemcpyHostToDevice),
+                           "hipMemcpy"));
         GSN_TRY(hip_status(hipMemcpy(T.ptr, t.data(), n1 * sizeof(E), hipMemcpyHostToDevice), "hipMemcpy"));
         GSN_TRY(hip_status(hipMemcpy(lo.ptr, vlo.data(), nlo * sizeof(E), hipMemcpyHostToDevice), "hipMemcpy"));
         GSN_TRY(hip_status(hipMemcpy(hi.ptr, vhi.data(), nhi * sizeof(E), hipMemcpyHostToDevice), "hipMemcpy"));
         num_passes = 2;
-        device_bytes = (n1 + nlo + nhi + n) * sizeof(E);
+        device_bytes = (n1 + nlo + nhi + (rad.count > 1 ? 2 : 1) * n + MIX_CS_ELEMS) * sizeof(E);
         if (a > 0) {
             // the rows: radix-2 transforms over w^n1 (a primitive 2^a-th root)
             const H wr = w.pow(n1);
@@ -590,9 +653,27 @@ struct MixedPlan : PlanBase {
     {
         GSN_TRY(take_injected_fault());
         const size_t n2 = (size_t)1 << a, n = (size_t)n1 * n2;
-        hipLaunchKernelGGL((mixed_odd_kernel<O, MI::MAXN1>), dim3((unsigned)n2), dim3(MIX_THREADS), 0, st,
-                           (E *)ws.ptr, (const E *)in, n1, (uint32_t)n2, (const E *)T.ptr, (const E *)lo.ptr,
-                           (const E *)hi.ptr, h, rad);
+        // passes alternate between the two workspaces so the last lands in
+        // ws (the caller's buffers are only read, then written by the
+        // transpose)
+        const E *src = (const E *)in;
+        uint32_t L = 1;
+        for (uint32_t pi = 0; pi < rad.count; ++pi) {
+            const bool last = pi + 1 == rad.count;
+            E *dst = (E *)(((rad.count - 1 - pi) & 1) ? ws2.ptr : ws.ptr);
+            const uint32_t q = rad.q[pi];
+            if (q == 7)
+                launch_mixed_pass<O, 7>(last, dst, src, n1, a, L, (const E *)T.ptr, (const E *)CS.ptr,
+                                        (const E *)lo.ptr, (const E *)hi.ptr, h, st);
+            else if (q == 5)
+                launch_mixed_pass<O, 5>(last, dst, src, n1, a, L, (const E *)T.ptr, (const E *)CS.ptr,
+                                        (const E *)lo.ptr, (const E *)hi.ptr, h, st);
+            else
+                launch_mixed_pass<O, 3>(last, dst, src, n1, a, L, (const E *)T.ptr, (const E *)CS.ptr,
+                                        (const E *)lo.ptr, (const E *)hi.ptr, h, st);
+            src = dst;
+            L *= q;
+        }
         GSN_TRY(hip_status(hipGetLastError(), "mixed-radix launch"));
         if (rows)
             GSN_TRY(rows->enqueue(st));
