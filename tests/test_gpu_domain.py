@@ -274,7 +274,8 @@ def mixed_roots(field, n1, a):
 @pytest.mark.parametrize("field,n1,a", [(g.FIELD_FP32, 3, 10), (g.FIELD_MNT4753_FR, 105, 6),
                                         (g.FIELD_MNT4753_FR, 525, 2), (g.FIELD_MNT4753_FR, 15, 9),
                                         (g.FIELD_MNT4753_FR, 7, 0), (g.FIELD_MNT6753_FR, 245, 3),
-                                        (g.FIELD_MNT6753_FR, 9, 5)])
+                                        (g.FIELD_MNT6753_FR, 9, 5), (g.FIELD_MNT6753_FR, 735, 2),
+                                        (g.FIELD_MNT4753_FR, 75, 6)])
 def test_mixed_radix_matches_oracle(orc, field, n1, a):
     """Plan.mixed (n = n1 2^a over w = w_{2^a} ROOT_ODD^(ODD_MAX/n1)) against
     the oracle's four-step restatement (itself pinned to the O(n^2) DFT),
