@@ -61,18 +61,25 @@ def folded_radix2(a, w, passes, log_w):
                 x.append([cur[j0 + c + jr * nR], cur[j0 + c + (jr + R // 2) * nR]])
             for s in range(log_r):
                 if s:
-                    # output e of thread t -> LDS element (t & m ? active : 0) + owner
+                    # output e of thread t -> LDS element owner, or active +
+                    # owner when bit tau of t is set
                     tau = lw + s - 1
                     m = 1 << tau
+                    where = lambda o, slot: active + o if slot else o
                     lds = [None] * (2 * active)
                     for t in range(active):
-                        slot = active if t & m else 0
                         # the partner is in the same 64-lane wavefront unless tau >= 6
                         assert ((t ^ m) >> 6 == t >> 6) == (tau < 6)
                         for e, owner in enumerate((t & ~m, t | m)):
-                            assert lds[slot + owner] is None
-                            lds[slot + owner] = x[t][e]
-                    x = [[lds[t], lds[active + t]] for t in range(active)]
+                            i = where(owner, t & m)
+                            # an exchange without a barrier touches only its
+                            # wavefront's slots: o and active + o for o of the
+                            # same wavefront (64 threads)
+                            if tau < 6:
+                                assert (i if i < active else i - active) >> 6 == t >> 6
+                            assert lds[i] is None
+                            lds[i] = x[t][e]
+                    x = [[lds[where(t, 0)], lds[where(t, 1)]] for t in range(active)]
                 for t in range(active):
                     c, jj = t & (W - 1), t >> lw
                     k = (j0 + c) & (Ns - 1)
