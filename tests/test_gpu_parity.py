@@ -58,7 +58,8 @@ def test_matches_oracle(orc, field, log_n):
     assert np.array_equal(back, x)
 
 
-@pytest.mark.parametrize("field,log_n", [(g.FIELD_MNT6753_FR, 18), (g.FIELD_FP32, 24), (g.FIELD_FP32, 26)])
+@pytest.mark.parametrize("field,log_n", [(g.FIELD_MNT6753_FR, 18), (g.FIELD_MNT6753_FR, 22), (g.FIELD_FP32, 24),
+                                         (g.FIELD_FP32, 26)])
 def test_roundtrip_and_linearity_full_size(field, log_n):
     say("full-size round trip", NAMES[field], log_n)
     p = g.MODULI[field]
