@@ -170,8 +170,9 @@ int gsn_plan_create_ext(gsn_plan **plan, int field, unsigned log_n, const void *
  * X_k = sum_j x_j w^(j k) over the primitive n-th root w = w_{2^a} w_{n1}
  * (the standard 2^a-th root times ROOT_ODD^(ODD_MAX / n1) of
  * gsn_field_params.h); inverse != 0: the inverse, with n^-1.  A four-step
- * factorisation: odd DFTs of the n1-element columns in LDS (radix 3, 5, 7),
- * radix-2 transforms of the rows, a transpose.  No counterpart in the
+ * factorisation: odd DFTs of the n1-element columns (radix 7, 5, 3 passes
+ * over all columns), radix-2 transforms of the rows, a transpose; the plan
+ * holds two workspaces of n elements.  No counterpart in the
  * reference (radix-2 only); it takes MNT4-753 Fr past 2^16 (up to
  * 2^15 * 525).  Executes with gsn_plan_execute (d_out may equal d_in). */
 int gsn_plan_create_mixed(gsn_plan **plan, int field, size_t n, int inverse);
