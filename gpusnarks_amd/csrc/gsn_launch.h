@@ -12,6 +12,11 @@ namespace gsn {
 enum : uint32_t {
     PASS_SCALE = 1u,  // multiply outputs by `scale` (n^-1 for the inverse)
     PASS_CANON = 2u,  // reduce outputs to [0, p)
+    // 753-bit passes after the first: multiply the first step's x0 by
+    // `scale`; the pass's step-0 records carry the same factor for x1, so
+    // the pass scales its inputs for one product per butterfly column
+    // instead of two per column at the end
+    PASS_SCALE_IN = 4u,
 };
 
 // Everything one Stockham pass of one transform needs.
@@ -75,8 +80,9 @@ void launch_scale(const typename F::elem_t *in, typename F::elem_t *out, typenam
 
 // Fills the step table of one 753-bit pass (see build_steps753_kernel):
 // nrec = Ns (2^log_r - 1) twiddle records in 14 planes of nrec quads.
+// scale0 != nullptr multiplies the step-0 records (the first Ns) by *scale0.
 template <class F>
 void launch_build_steps753(uint4 *out, uint32_t nrec, const E753 *hi, const TW753 *lo, uint32_t h, uint32_t log_n,
-                           uint32_t log_ns, uint32_t log_r, hipStream_t stream);
+                           uint32_t log_ns, uint32_t log_r, const TW753 *scale0, hipStream_t stream);
 
 }  // namespace gsn

@@ -352,8 +352,11 @@ static int build_tables_753(Plan &plan, const H &w, unsigned log_n, size_t n)
         const uint32_t nrec = (uint32_t)((((size_t)1 << lr) - 1) << log_ns);
         GSN_TRY(plan.step_tabs[i].alloc((size_t)nrec * sizeof(TW753)));
         plan.step_nrec[i] = nrec;
+        // the inverse's n^-1 rides on the last pass's step-0 records when
+        // there is more than one pass (PASS_SCALE_IN)
+        const bool scale0 = plan.inverse && P_ >= 2 && i == P_ - 1;
         launch_build_steps753<F>((uint4 *)plan.step_tabs[i].ptr, nrec, (const E753 *)dhi.ptr, (const TW753 *)dlo.ptr,
-                                 h, log_n, log_ns, lr, 0);
+                                 h, log_n, log_ns, lr, scale0 ? &plan.scale : nullptr, 0);
         GSN_TRY(hip_status(hipGetLastError(), "build_steps753 launch"));
         log_ns += lr;
     }

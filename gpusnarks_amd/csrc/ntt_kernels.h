@@ -378,8 +378,11 @@ __device__ __forceinline__ void tile753_body(const PassArgs<F> &a, int log_r, in
         const int jr = log_r > 1 ? (int)(__builtin_bitreverse32((uint32_t)jj) >> (33 - log_r)) : 0;
         x[1] = F::load(a.in, j + (size_t)(jr + half) * nR);
         x[0] = F::load(a.in, j + (size_t)jr * nR);
-        if (log_ns != 0)
+        if (log_ns != 0) {
             x[1] = F::mul(x[1], F::load_tw_planes(tab, planes, k));
+            if (a.flags & PASS_SCALE_IN)
+                x[0] = F::mul(x[0], a.scale);
+        }
         w = step_tw(1, x[1].v[23]);
         const elem s0 = F::add(x[0], x[1]);
         x[1] = F::sub(x[0], x[1]);
@@ -510,7 +513,8 @@ __global__ __launch_bounds__(1 << (LOG_TILE - 1)) __attribute__((amdgpu_waves_pe
 // w_n^e = hi[e >> h] * lo[e & (2^h - 1)] reduced to [0, p).
 template <class F>
 __global__ __launch_bounds__(256) void build_steps753_kernel(uint4 *out, uint32_t nrec, const E753 *hi, const TW753 *lo,
-                                                            uint32_t h, uint32_t log_n, uint32_t log_ns, uint32_t log_r)
+                                                            uint32_t h, uint32_t log_n, uint32_t log_ns, uint32_t log_r,
+                                                            TW753 scale0, int use_scale0)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nrec)
@@ -519,7 +523,10 @@ __global__ __launch_bounds__(256) void build_steps753_kernel(uint4 *out, uint32_
     const uint32_t jx = i - (((1u << s) - 1u) << log_ns);
     const uint32_t k = jx >> s, kk = jx & ((1u << s) - 1u);
     const uint32_t e = (k + (kk << log_ns)) << (log_n - log_ns - s - 1u);
-    const TW753 r = F::make_tw(F::canon(F::mul(hi[e >> h], lo[e & ((1u << h) - 1u)])));
+    E753 v = F::canon(F::mul(hi[e >> h], lo[e & ((1u << h) - 1u)]));
+    if (use_scale0 && s == 0)
+        v = F::canon(F::mul(v, scale0));
+    const TW753 r = F::make_tw(v);
     const uint4 *q = reinterpret_cast<const uint4 *>(&r);
 #pragma unroll
     for (int p = 0; p < 14; ++p)

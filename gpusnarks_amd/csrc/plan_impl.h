@@ -185,7 +185,10 @@ struct PlanImpl final : PlanBase {
             }
             a.log_n = log_n;
             a.log_ns = log_ns;
-            a.flags = (i == P - 1) ? (PASS_CANON | (inverse ? PASS_SCALE : 0u)) : 0u;
+            // the inverse's n^-1: for the 753-bit field with two or more
+            // passes, on the last pass's inputs (its step-0 records carry it)
+            const uint32_t scale_flag = !IS32 && P >= 2 ? PASS_SCALE_IN : PASS_SCALE;
+            a.flags = (i == P - 1) ? (PASS_CANON | (inverse ? scale_flag : 0u)) : 0u;
             a.scale = scale;
             jb.log_r = (uint32_t)c.log_r;
             jb.log_w = (uint32_t)c.log_w;
