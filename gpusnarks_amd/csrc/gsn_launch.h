@@ -27,6 +27,9 @@ struct PassArgs {
     const typename F::tw_t *tw_col;
     const typename F::tw_t *tw_pass;
     const typename F::tw_t *tw_pass_lo;
+    // 32-bit passes with a two-level pass table: w_{Ns R}^(jj k) at
+    // jj Ns + k (jj < R / E, k < Ns), read by adjacent lanes at adjacent k
+    const typename F::tw_t *tw_pass_a;
     uint32_t log_n;
     uint32_t log_ns;
     uint32_t col_shift;   // column table index = exponent << col_shift

@@ -208,7 +208,7 @@ __device__ __forceinline__ void pass32_body(const PassArgs<F32> &a, uint32_t blk
             const uint32_t kc = (j0 + (uint32_t)(i & (W - 1))) & ns_mask;
             colBC[i] = twx((i < W ? (uint32_t)(R / E) : (uint32_t)(R / E0)) * kc);
         }
-        F::tw_t tg = twx((uint32_t)jj * k);
+        F::tw_t tg = a.tw_pass_a ? ldtw32(a.tw_pass_a, ((uint32_t)jj << a.log_ns) + k) : twx((uint32_t)jj * k);
         __syncthreads();
         const F::tw_t tB = colBC[c], tC = colBC[W + c];
 #pragma unroll

@@ -105,7 +105,7 @@ struct PlanImpl final : PlanBase {
     std::vector<uint32_t> step_nrec;
     // 32-bit, per pass: column table w_R^x (x < R) and pass table in units
     // of w_{Ns R} -- direct, or hi/lo halves of a two-level split
-    std::vector<DeviceBuffer> col_tabs, pass_tabs, pass_los;
+    std::vector<DeviceBuffer> col_tabs, pass_tabs, pass_los, pass_as;
     std::vector<uint32_t> pass_lo_bits;
     DeviceBuffer scratch[2];
     tw scale{};
@@ -126,7 +126,7 @@ struct PlanImpl final : PlanBase {
             return GSN_OK;
         GSN_TRY(build_tables(w, n));
         device_bytes = 0;
-        for (auto *v : {&col_tabs, &pass_tabs, &pass_los, &step_tabs})
+        for (auto *v : {&col_tabs, &pass_tabs, &pass_los, &pass_as, &step_tabs})
             for (auto &b : *v)
                 device_bytes += b.bytes;
         return GSN_OK;
@@ -170,6 +170,7 @@ struct PlanImpl final : PlanBase {
                 a.col_shift = 0;
                 a.tw_pass = (const tw *)pass_tabs[i].ptr;
                 a.tw_pass_lo = (const tw *)pass_los[i].ptr;  // nullptr: direct table
+                a.tw_pass_a = (const tw *)pass_as[i].ptr;    // nullptr: from the pass table
                 a.lo_bits = pass_lo_bits[i];
                 a.pass_shift = 0;
             } else {
@@ -180,6 +181,7 @@ struct PlanImpl final : PlanBase {
                 a.col_shift = 0;
                 a.tw_pass = nullptr;
                 a.tw_pass_lo = nullptr;
+                a.tw_pass_a = nullptr;
                 a.lo_bits = 0;
                 a.pass_shift = 0;
             }
