@@ -418,23 +418,26 @@ def bench_batched_same(log_n=18, count=16, reps=5):
             "launches_batched": batch.num_launches, "bit_exact_vs_plain": bool(same)}
 
 
-def bench_serving(requests=160, rates=(0.3, 0.7, 1.1), seed=0x5EED0004):
+def bench_serving(requests=1000, rates=(0.3, 0.7, 1.1), seed=0x5EED0004):
     import gpusnarks_amd as g
     rng = random.Random(seed)
     reqs = []
     for k in range(requests):
         field = g.FIELD_MNT6753_FR if rng.random() < 0.7 else g.FIELD_FP32
         reqs.append((field, rng.randint(12, 20), rng.random() < 0.5))
-    bufs = []
+    # one input per (field, size) class, one output buffer per request (the
+    # transforms run out of place, so requests never share a buffer they write)
+    ins = {}
     for k, (field, log_n, inv) in enumerate(reqs):
-        x = inputs(field, log_n, 0x5E4E + k)
-        bufs.append((g.DeviceArray.from_numpy(x), x))
+        if (field, log_n) not in ins:
+            ins[(field, log_n)] = g.DeviceArray.from_numpy(inputs(field, log_n, 0x5E4E + k))
+    bufs = [(ins[(field, log_n)], g.DeviceArray(ins[(field, log_n)].nbytes)) for field, log_n, _ in reqs]
     plans = {key: g.Plan(*key[:2], inverse=key[2]) for key in set(reqs)}
     s = g.Stream()
     # plain capacity: back-to-back one-at-a-time service time
     t0 = time.perf_counter()
-    for key, (d, _) in zip(reqs, bufs):
-        plans[key].execute(d.ptr, d.ptr, s.handle)
+    for key, (d, o) in zip(reqs, bufs):
+        plans[key].execute(d.ptr, o.ptr, s.handle)
         s.synchronize()
     busy = time.perf_counter() - t0
     capacity = requests / busy  # requests/s
@@ -445,9 +448,9 @@ def bench_serving(requests=160, rates=(0.3, 0.7, 1.1), seed=0x5EED0004):
     # warm-up: one request per (field, size, direction) builds the plans'
     # twiddle tables before anything is timed
     seen = {}
-    for key, (d, _) in zip(reqs, bufs):
+    for key, (d, o) in zip(reqs, bufs):
         if key not in seen:
-            seen[key] = srv.submit_device(key[0], key[1], d.ptr, d.ptr, inverse=key[2])
+            seen[key] = srv.submit_device(key[0], key[1], d.ptr, o.ptr, inverse=key[2])
     for t in seen.values():
         srv.wait(t)
     for frac in rates:
@@ -456,22 +459,22 @@ def bench_serving(requests=160, rates=(0.3, 0.7, 1.1), seed=0x5EED0004):
         # one-at-a-time plain dispatch (FIFO, each request runs alone)
         lat_plain = []
         t_start = time.perf_counter()
-        for a, key, (d, _) in zip(arrivals, reqs, bufs):
+        for a, key, (d, o) in zip(arrivals, reqs, bufs):
             now = time.perf_counter() - t_start
             if now < a:
                 time.sleep(a - now)
-            plans[key].execute(d.ptr, d.ptr, s.handle)
+            plans[key].execute(d.ptr, o.ptr, s.handle)
             s.synchronize()
             lat_plain.append(time.perf_counter() - t_start - a)
         span_plain = time.perf_counter() - t_start
         # served: continuous batching
         tickets = []
         t_start = time.perf_counter()
-        for a, (field, log_n, inv), (d, _) in zip(arrivals, reqs, bufs):
+        for a, (field, log_n, inv), (d, o) in zip(arrivals, reqs, bufs):
             now = time.perf_counter() - t_start
             if now < a:
                 time.sleep(a - now)
-            tickets.append(srv.submit_device(field, log_n, d.ptr, d.ptr, inverse=inv))
+            tickets.append(srv.submit_device(field, log_n, d.ptr, o.ptr, inverse=inv))
         for t in tickets:
             srv.wait(t)
         span_srv = time.perf_counter() - t_start
