@@ -210,3 +210,23 @@ def test_every_tile_shape_matches_oracle(orc, field, log_n, schedules):
             assert np.array_equal(g.best_ifft(got, om, field), x), sched
     finally:
         g.tuning_set(field, log_n, before)
+
+
+@pytest.mark.parametrize("log_n,schedule", [(22, [(9, 4, 4), (8, 4, 4), (5, 4, 4)])])
+def test_fp32_direct_pass_table_matches_oracle(orc, log_n, schedule):
+    """32-bit passes past 2^17 columns read the pass-twiddle base w^(jj k)
+    from a direct [jj][k] table: a last pass at Ns = 2^17, bit-exact against
+    the oracle, forward and inverse."""
+    field = g.FIELD_FP32
+    say("direct pass table", log_n, schedule)
+    x = inputs(field, log_n, 0xD1EC7 + log_n)
+    om = g.root_of_unity(field, log_n)
+    want = orc.serial_fft(field, x, om)
+    before = g.tuning_get(field, log_n)
+    try:
+        g.tuning_set(field, log_n, schedule)
+        got = g.best_fft(x, om, field)
+        assert np.array_equal(got, want)
+        assert np.array_equal(g.best_ifft(got, om, field), x)
+    finally:
+        g.tuning_set(field, log_n, before)
