@@ -72,6 +72,11 @@ struct Ops32 {
     __device__ static elem add(elem a, elem b) { return F32::add(a, b); }
     __device__ static elem sub(elem a, elem b) { return F32::sub(a, b); }
     __device__ static elem mul(elem a, elem b) { return F32::mul(a, F32::tw_t{b, b * Fp32Params::PINV}); }
+    // product with a constant's precomputed record
+    using tw = F32::tw_t;
+    __device__ static elem mul_tw(elem a, tw t) { return F32::mul(a, t); }
+    __device__ static elem mul_tw2(elem a, tw t, tw u) { return F32::mul(F32::mul(a, t), u); }
+    __device__ static tw load_tw(const tw *p, size_t i) { return F32::load_tw(p, i); }
     __device__ static elem one() { return Fp32Params::ONE; }
     __device__ static elem zero() { return 0u; }
     __device__ static elem load(const elem *p, size_t i) { return p[i]; }
@@ -84,6 +89,14 @@ struct Ops753 {
     __device__ static elem add(const elem &a, const elem &b) { return F::add_mod(a, b); }
     __device__ static elem sub(const elem &a, const elem &b) { return F::sub_mod(a, b); }
     __device__ static elem mul(const elem &a, const elem &b) { return F::mont_mul(a, b); }
+    // product with a constant's precomputed record (quotient estimate),
+    // reduced to [0, p)
+    using tw = TW753;
+    __device__ static elem mul_tw(const elem &a, const tw &t) { return F::canon(F::mul(a, t)); }
+    // by two constants, one reduction: the first product, below 3p, is a
+    // valid input of the second
+    __device__ static elem mul_tw2(const elem &a, const tw &t, const tw &u) { return F::canon(F::mul(F::mul(a, t), u)); }
+    __device__ static tw load_tw(const tw *p, size_t i) { return F::load_tw(p, i); }
     __device__ static elem one()
     {
         elem o;
@@ -209,9 +222,10 @@ __global__ __launch_bounds__(256) void ext_inv_kernel(typename F::elem_t *a, con
 // radix q over stride L twiddles input r of group j by w_{Lq}^(r (j mod L))
 // and writes output s to position (j / L) L q + (j mod L) + s L.  Position
 // i of column j2 lives at i n2 + j2 in every buffer.  The last pass also
-// multiplies by w^(j2 k1) and so leaves Y row-major by k1.  Products are
-// general Montgomery products against tables of Montgomery forms:
-// T[e] = (w^n2)^e for e < n1, and w^e = lo[e mod 2^h] hi[e >> h].
+// multiplies by w^(j2 k1) and so leaves Y row-major by k1.  Every product
+// is by a constant, so its tables hold the transform's twiddle records
+// (precomputed quotient for the 753-bit fields, reduced to [0, p) after each
+// product): T[e] = (w^n2)^e for e < n1, and w^e = lo[e mod 2^h] hi[e >> h].
 //
 // Each q-point DFT folds its inputs in pairs: with u = w_q, a_m = x_m +
 // x_{q-m} and b_m = x_m - x_{q-m} (m = 1..(q-1)/2),
@@ -228,11 +242,10 @@ __host__ __device__ constexpr uint32_t mix_cs_offset(uint32_t q) { return q == 3
 constexpr uint32_t MIX_CS_ELEMS = 30;
 
 template <class O, int Q, bool LAST>
-__global__ __launch_bounds__(256) void mixed_pass_kernel(typename O::elem *dst, const typename O::elem *src,
-                                                         uint32_t n1, uint32_t log_n2, uint32_t L,
-                                                         const typename O::elem *T, const typename O::elem *CS,
-                                                         const typename O::elem *lo, const typename O::elem *hi,
-                                                         uint32_t h)
+__device__ __forceinline__ void mixed_pass_body(typename O::elem *dst, const typename O::elem *src, uint32_t n1,
+                                                uint32_t log_n2, uint32_t L, const typename O::tw *T,
+                                                const typename O::tw *CS, const typename O::tw *lo,
+                                                const typename O::tw *hi, uint32_t h)
 {
     using E = typename O::elem;
     constexpr int H = (Q - 1) / 2;
@@ -243,7 +256,7 @@ __global__ __launch_bounds__(256) void mixed_pass_kernel(typename O::elem *dst, 
     const size_t j2 = tid & (((size_t)1 << log_n2) - 1);
     const uint32_t j = (uint32_t)(tid >> log_n2);
     const uint32_t k = j % L, jb = j / L;
-    const E *C = CS + mix_cs_offset(Q), *S = C + Q;
+    const typename O::tw *C = CS + mix_cs_offset(Q), *S = C + Q;
     E x[Q];
 #pragma unroll
     for (int r = 0; r < Q; ++r)
@@ -251,14 +264,14 @@ __global__ __launch_bounds__(256) void mixed_pass_kernel(typename O::elem *dst, 
     if (k != 0)  // r k span < n1: w_{Lq}^(r k) = T[r k span]
 #pragma unroll
         for (int r = 1; r < Q; ++r)
-            x[r] = O::mul(x[r], O::load(T, (size_t)r * k * span));
+            x[r] = O::mul_tw(x[r], O::load_tw(T, (size_t)r * k * span));
     const uint32_t pos0 = jb * L * Q + k;
     auto put = [&](uint32_t s, const E &v) {
         const uint32_t pos = pos0 + s * L;
         if (LAST) {
             const uint64_t e = (uint64_t)j2 * pos;
             O::store(dst, ((size_t)pos << log_n2) + j2,
-                     O::mul(O::mul(v, O::load(lo, (size_t)(e & ((1ull << h) - 1)))), O::load(hi, (size_t)(e >> h))));
+                     O::mul_tw2(v, O::load_tw(lo, (size_t)(e & ((1ull << h) - 1))), O::load_tw(hi, (size_t)(e >> h))));
         } else {
             O::store(dst, ((size_t)pos << log_n2) + j2, v);
         }
@@ -274,12 +287,12 @@ __global__ __launch_bounds__(256) void mixed_pass_kernel(typename O::elem *dst, 
     put(0, s0);
 #pragma unroll
     for (int kk = 1; kk <= H; ++kk) {
-        E U = O::mul(x[1], O::load(C, (uint32_t)kk % Q));
-        E V = O::mul(x[Q - 1], O::load(S, (uint32_t)kk % Q));
+        E U = O::mul_tw(x[1], O::load_tw(C, (uint32_t)kk % Q));
+        E V = O::mul_tw(x[Q - 1], O::load_tw(S, (uint32_t)kk % Q));
 #pragma unroll
         for (int m = 2; m <= H; ++m) {
-            U = O::add(U, O::mul(x[m], O::load(C, (uint32_t)(m * kk) % Q)));
-            V = O::add(V, O::mul(x[Q - m], O::load(S, (uint32_t)(m * kk) % Q)));
+            U = O::add(U, O::mul_tw(x[m], O::load_tw(C, (uint32_t)(m * kk) % Q)));
+            V = O::add(V, O::mul_tw(x[Q - m], O::load_tw(S, (uint32_t)(m * kk) % Q)));
         }
         const E base = O::add(x[0], U);
         put((uint32_t)kk, O::add(base, V));
@@ -287,19 +300,47 @@ __global__ __launch_bounds__(256) void mixed_pass_kernel(typename O::elem *dst, 
     }
 }
 
+template <class O, int Q, bool LAST>
+__global__ __launch_bounds__(256) void mixed_pass_kernel(typename O::elem *dst, const typename O::elem *src,
+                                                         uint32_t n1, uint32_t log_n2, uint32_t L,
+                                                         const typename O::tw *T, const typename O::tw *CS,
+                                                         const typename O::tw *lo, const typename O::tw *hi,
+                                                         uint32_t h)
+{
+    mixed_pass_body<O, Q, LAST>(dst, src, n1, log_n2, L, T, CS, lo, hi, h);
+}
+// radix 3 holds three elements: held to 256 registers it runs two
+// wavefronts per SIMD (radix 5 and 7 need more registers than that)
+template <class O, bool LAST>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void mixed_pass3_kernel(
+    typename O::elem *dst, const typename O::elem *src, uint32_t n1, uint32_t log_n2, uint32_t L,
+    const typename O::tw *T, const typename O::tw *CS, const typename O::tw *lo, const typename O::tw *hi, uint32_t h)
+{
+    mixed_pass_body<O, 3, LAST>(dst, src, n1, log_n2, L, T, CS, lo, hi, h);
+}
+
 template <class O, int Q>
 void launch_mixed_pass(bool last, typename O::elem *dst, const typename O::elem *src, uint32_t n1, uint32_t log_n2,
-                       uint32_t L, const typename O::elem *T, const typename O::elem *CS,
-                       const typename O::elem *lo, const typename O::elem *hi, uint32_t h, hipStream_t st)
+                       uint32_t L, const typename O::tw *T, const typename O::tw *CS, const typename O::tw *lo,
+                       const typename O::tw *hi, uint32_t h, hipStream_t st)
 {
     const size_t threads = ((size_t)(n1 / Q)) << log_n2;
     const dim3 grid((unsigned)((threads + 255) / 256)), block(256);
-    if (last)
-        hipLaunchKernelGGL((mixed_pass_kernel<O, Q, true>), grid, block, 0, st, dst, src, n1, log_n2, L, T, CS, lo,
-                           hi, h);
-    else
-        hipLaunchKernelGGL((mixed_pass_kernel<O, Q, false>), grid, block, 0, st, dst, src, n1, log_n2, L, T, CS,
-                           lo, hi, h);
+    if constexpr (Q == 3) {
+        if (last)
+            hipLaunchKernelGGL((mixed_pass3_kernel<O, true>), grid, block, 0, st, dst, src, n1, log_n2, L, T, CS, lo,
+                               hi, h);
+        else
+            hipLaunchKernelGGL((mixed_pass3_kernel<O, false>), grid, block, 0, st, dst, src, n1, log_n2, L, T, CS,
+                               lo, hi, h);
+    } else {
+        if (last)
+            hipLaunchKernelGGL((mixed_pass_kernel<O, Q, true>), grid, block, 0, st, dst, src, n1, log_n2, L, T, CS,
+                               lo, hi, h);
+        else
+            hipLaunchKernelGGL((mixed_pass_kernel<O, Q, false>), grid, block, 0, st, dst, src, n1, log_n2, L, T, CS,
+                               lo, hi, h);
+    }
 }
 
 // out[k1 + n1 k2] = ws[k1 n2 + k2]
@@ -564,15 +605,13 @@ struct MixedPlan : PlanBase {
     using MI = MixInfo<FID>;
     using O = typename MI::O;
     using E = typename O::elem;
+    using TW = typename O::tw;
     uint32_t n1 = 0, a = 0, h = 0;
     MixRadices rad{};
-    DeviceBuffer T, lo, hi, ws, ws2, CS;
+    DeviceBuffer T, lo, hi, ws, ws2, CS;  // T, lo, hi, CS: records of constants
     std::unique_ptr<BatchImpl> rows;
 
-    static void put(std::vector<E> &v, size_t i, const H &x)
-    {
-        std::memcpy(&v[i], x.im_rep, sizeof(E));
-    }
+    static void put(std::vector<TW> &v, size_t i, const H &x) { v[i] = record<FID>(x); }
 
     int init(uint32_t n1_, uint32_t a_, int inv)
     {
@@ -590,14 +629,14 @@ struct MixedPlan : PlanBase {
         const H w = w2 * MI::root_odd().pow(MI::ODD_MAX / n1);
         const H base = inv ? w.mul_inv() : w;
         // T[e] = (base^n2)^e, e < n1
-        std::vector<E> t(n1);
+        std::vector<TW> t(n1);
         const H w1 = base.pow(n2);
         H acc = H::one();
         for (uint32_t e = 0; e < n1; ++e, acc = acc * w1)
             put(t, e, acc);
         // symmetric DFT constants per radix q: u = w1^(n1/q), C[e] = (u^e +
         // u^-e)/2, S[e] = (u^e - u^-e)/2
-        std::vector<E> cs(MIX_CS_ELEMS);
+        std::vector<TW> cs(MIX_CS_ELEMS);
         const H half = H(2u).mul_inv();
         for (uint32_t q : {3u, 5u, 7u}) {
             if (n1 % q)
@@ -613,27 +652,27 @@ struct MixedPlan : PlanBase {
         while (((size_t)1 << (2 * h)) < n)
             ++h;
         const size_t nlo = (size_t)1 << h, nhi = (n + nlo - 1) >> h;
-        std::vector<E> vlo(nlo), vhi(nhi);
+        std::vector<TW> vlo(nlo), vhi(nhi);
         acc = H::one();
         for (size_t e = 0; e < nlo; ++e, acc = acc * base)
             put(vlo, e, acc);
         H hacc = inv ? H(n1).mul_inv() : H::one();
         for (size_t y = 0; y < nhi; ++y, hacc = hacc * acc)
             put(vhi, y, hacc);
-        GSN_TRY(T.alloc(n1 * sizeof(E)));
-        GSN_TRY(lo.alloc(nlo * sizeof(E)));
-        GSN_TRY(hi.alloc(nhi * sizeof(E)));
+        GSN_TRY(T.alloc(n1 * sizeof(TW)));
+        GSN_TRY(lo.alloc(nlo * sizeof(TW)));
+        GSN_TRY(hi.alloc(nhi * sizeof(TW)));
         GSN_TRY(ws.alloc(n * sizeof(E)));
         if (rad.count > 1)
             GSN_TRY(ws2.alloc(n * sizeof(E)));
-        GSN_TRY(CS.alloc(MIX_CS_ELEMS * sizeof(E)));
-        GSN_TRY(hip_status(hipMemcpy(CS.ptr, cs.data(), MIX_CS_ELEMS * sizeof(E), hipMemcpyHostToDevice),
+        GSN_TRY(CS.alloc(MIX_CS_ELEMS * sizeof(TW)));
+        GSN_TRY(hip_status(hipMemcpy(CS.ptr, cs.data(), MIX_CS_ELEMS * sizeof(TW), hipMemcpyHostToDevice),
                            "hipMemcpy"));
-        GSN_TRY(hip_status(hipMemcpy(T.ptr, t.data(), n1 * sizeof(E), hipMemcpyHostToDevice), "hipMemcpy"));
-        GSN_TRY(hip_status(hipMemcpy(lo.ptr, vlo.data(), nlo * sizeof(E), hipMemcpyHostToDevice), "hipMemcpy"));
-        GSN_TRY(hip_status(hipMemcpy(hi.ptr, vhi.data(), nhi * sizeof(E), hipMemcpyHostToDevice), "hipMemcpy"));
+        GSN_TRY(hip_status(hipMemcpy(T.ptr, t.data(), n1 * sizeof(TW), hipMemcpyHostToDevice), "hipMemcpy"));
+        GSN_TRY(hip_status(hipMemcpy(lo.ptr, vlo.data(), nlo * sizeof(TW), hipMemcpyHostToDevice), "hipMemcpy"));
+        GSN_TRY(hip_status(hipMemcpy(hi.ptr, vhi.data(), nhi * sizeof(TW), hipMemcpyHostToDevice), "hipMemcpy"));
         num_passes = 2;
-        device_bytes = (n1 + nlo + nhi + (rad.count > 1 ? 2 : 1) * n + MIX_CS_ELEMS) * sizeof(E);
+        device_bytes = (n1 + nlo + nhi + MIX_CS_ELEMS) * sizeof(TW) + (rad.count > 1 ? 2 : 1) * n * sizeof(E);
         if (a > 0) {
             // the rows: radix-2 transforms over w^n1 (a primitive 2^a-th root)
             const H wr = w.pow(n1);
@@ -663,14 +702,14 @@ struct MixedPlan : PlanBase {
             E *dst = (E *)(((rad.count - 1 - pi) & 1) ? ws2.ptr : ws.ptr);
             const uint32_t q = rad.q[pi];
             if (q == 7)
-                launch_mixed_pass<O, 7>(last, dst, src, n1, a, L, (const E *)T.ptr, (const E *)CS.ptr,
-                                        (const E *)lo.ptr, (const E *)hi.ptr, h, st);
+                launch_mixed_pass<O, 7>(last, dst, src, n1, a, L, (const TW *)T.ptr, (const TW *)CS.ptr,
+                                        (const TW *)lo.ptr, (const TW *)hi.ptr, h, st);
             else if (q == 5)
-                launch_mixed_pass<O, 5>(last, dst, src, n1, a, L, (const E *)T.ptr, (const E *)CS.ptr,
-                                        (const E *)lo.ptr, (const E *)hi.ptr, h, st);
+                launch_mixed_pass<O, 5>(last, dst, src, n1, a, L, (const TW *)T.ptr, (const TW *)CS.ptr,
+                                        (const TW *)lo.ptr, (const TW *)hi.ptr, h, st);
             else
-                launch_mixed_pass<O, 3>(last, dst, src, n1, a, L, (const E *)T.ptr, (const E *)CS.ptr,
-                                        (const E *)lo.ptr, (const E *)hi.ptr, h, st);
+                launch_mixed_pass<O, 3>(last, dst, src, n1, a, L, (const TW *)T.ptr, (const TW *)CS.ptr,
+                                        (const TW *)lo.ptr, (const TW *)hi.ptr, h, st);
             src = dst;
             L *= q;
         }
