@@ -61,7 +61,7 @@ def inputs(field, log_n, seed):
     return synth.uniform_residues(seed, 1 << log_n, g.MODULI[field], g.NLIMBS[field])
 
 
-def time_plan(field, log_n, steps, warmup, inverse=False):
+def time_plan(field, log_n, steps, warmup, inverse=False, want_output=False):
     """Device-resident transform, timed with HIP events on its own stream."""
     import gpusnarks_amd as g
     x = inputs(field, log_n, 0x5EED0000 + log_n)
@@ -83,6 +83,8 @@ def time_plan(field, log_n, steps, warmup, inverse=False):
     out = d_out.download(x.shape)
     ref = g.best_fft(x, None, field, inverse=inverse)
     assert np.array_equal(out, ref), "device-resident result differs from the host path"
+    if want_output:
+        return ms / steps, plan.num_passes, out
     return ms / steps, plan.num_passes
 
 
@@ -186,9 +188,30 @@ def native_oracle():
     return _NATIVE_ORACLE[0]
 
 
-def cpu_baseline(log_n, threads, field=None, serial=True, j_sample=0):
+def cpu_full_run(log_n, threads, field, gpu_out):
+    """The reference harness's host path in full (test/main.cpp:66-84): the
+    oracle's _basic_parallel_radix2_FFT_inner with log_cpus = 10 on `threads`
+    cores, wall clock, and its output compared element for element with the
+    GPU's transform of the same input."""
+    from tests.oracle_lib import Oracle
+    import gpusnarks_amd as g
+    so, build_flags = native_oracle()
+    orc = Oracle(so)
+    x = inputs(field, log_n, 0x5EED0000 + log_n)
+    om = g.root_of_unity(field, log_n)
+    t0 = time.perf_counter()
+    y = orc.parallel_fft(field, x, om, 10, threads=threads)
+    sec = time.perf_counter() - t0
+    return {"log2_n": log_n, "seconds": sec, "butterflies_per_s": bfly(log_n) / sec, "cores": threads,
+            "bit_exact_vs_gpu": bool(np.array_equal(y, gpu_out)), "build": build_flags,
+            "note": "full run, not a sample: phase 1 (1024 partial sums), 1024 sub-transforms, scatter"}
+
+
+def cpu_baseline(log_n, threads, field=None, serial=True, j_sample=0, gpu_out=None):
     """Oracle (test/fft_host.h restated) on the host: the reference
-    harness's parallel radix-2 FFT with log_cpus = 10, bounded sample."""
+    harness's parallel radix-2 FFT with log_cpus = 10, bounded sample.  With
+    `gpu_out`, the serial transform's output (computed in full) is compared
+    element for element with the GPU's (test/main.cpp:80-84)."""
     import ctypes
     from tests.oracle_lib import Oracle
     import gpusnarks_amd as g
@@ -236,10 +259,12 @@ def cpu_baseline(log_n, threads, field=None, serial=True, j_sample=0):
     if serial:
         # and the O(n log n) serial transform on one core, in full
         t0 = time.perf_counter()
-        orc.serial_fft(field, x, om)
+        y = orc.serial_fft(field, x, om)
         t_serial = time.perf_counter() - t0
         out["serial_radix2_1core_butterflies_per_s"] = bfly(log_n) / t_serial
         out["serial_radix2_1core_seconds"] = t_serial
+        if gpu_out is not None:
+            out["bit_exact_vs_gpu"] = bool(np.array_equal(y, gpu_out))
     return out
 
 
@@ -319,23 +344,13 @@ def bench_configs(threads, skip_cpu):
     return out
 
 
-def ragged_workload(seed=0x5EED0003, count=256):
-    import gpusnarks_amd as g
-    rng = random.Random(seed)
-    items = []
-    for i in range(count):
-        field = g.FIELD_MNT6753_FR if i < count * 3 // 4 else g.FIELD_FP32
-        items.append((field, rng.randint(10, 18), rng.random() < 0.5))
-    rng.shuffle(items)
-    return items
-
-
 def bench_batched(reps=5):
     import gpusnarks_amd as g
-    items = ragged_workload()
+    from gpusnarks_amd import synth
+    items = synth.ragged_workload()
     bufs = []
     for k, (field, log_n, inv) in enumerate(items):
-        x = inputs(field, log_n, 0xBA7C4 + k)
+        x = inputs(field, log_n, synth.RAGGED_INPUT_SEED + k)
         bufs.append((g.DeviceArray.from_numpy(x), g.DeviceArray(x.nbytes), x))
     total_bfly = sum(bfly(l) for _, l, _ in items)
     s = g.Stream()
@@ -420,11 +435,9 @@ def bench_batched_same(log_n=18, count=16, reps=5):
 
 def bench_serving(requests=1000, rates=(0.3, 0.7, 1.1), seed=0x5EED0004):
     import gpusnarks_amd as g
-    rng = random.Random(seed)
-    reqs = []
-    for k in range(requests):
-        field = g.FIELD_MNT6753_FR if rng.random() < 0.7 else g.FIELD_FP32
-        reqs.append((field, rng.randint(12, 20), rng.random() < 0.5))
+    from gpusnarks_amd import synth
+    reqs = synth.serving_requests(seed, requests)
+    rng = random.Random(seed + 1)  # arrival times
     # one input per (field, size) class, one output buffer per request (the
     # transforms run out of place, so requests never share a buffer they write)
     ins = {}
@@ -532,7 +545,7 @@ def main():
         print(json.dumps({"field": args.field, "log2_n": log_n, "ms": ms, "passes": passes,
                           "butterflies_per_s": bfly(log_n) / (ms * 1e-3)}))
         return
-    ms, passes = time_plan(field, log_n, args.steps, args.warmup)
+    ms, passes, gpu_out = time_plan(field, log_n, args.steps, args.warmup, want_output=True)
     value = bfly(log_n) / (ms * 1e-3)
     ms_launch = ms / passes
     mads_per_launch = bfly(log_n) / passes * MADS_PER_BFLY_753
@@ -572,7 +585,14 @@ def main():
     if not args.skip_cpu:
         threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
         log("cpu baseline on %d threads" % threads)
-        line["cpu_baseline"] = cpu_baseline(log_n, threads, j_sample=512)
+        line["cpu_baseline"] = cpu_baseline(log_n, threads, j_sample=512, gpu_out=gpu_out)
+        line["bit_exact_vs_cpu"] = line["cpu_baseline"].get("bit_exact_vs_gpu")
+        # the same harness path run in full at 2^16, beside its sampled estimate
+        _, _, out16 = time_plan(field, 16, 3, 1, want_output=True)
+        full16 = cpu_full_run(16, threads, field, out16)
+        est16 = cpu_baseline(16, threads, field, serial=False)
+        full16["sampled_estimate_seconds"] = est16["seconds_extrapolated"]
+        line["cpu_baseline"]["full_run_n16"] = full16
     if not args.skip_extras:
         log("fp32 n=2^26")
         ms32, p32 = time_plan(g.FIELD_FP32, 26, 20, 3)

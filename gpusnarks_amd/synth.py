@@ -66,3 +66,38 @@ def to_int(limbs_row):
 
 def from_int(x, nlimbs):
     return np.array([(x >> (32 * i)) & 0xFFFFFFFF for i in range(nlimbs)], dtype=np.uint32)
+
+
+# Field ids of the C ABI (include/gpusnarks.h); repeated here so this module
+# imports without the library.
+_FP32, _MNT6 = 0, 2
+
+
+def ragged_workload(seed=0x5EED0003, count=256):
+    """The batched ragged workload: `count` (field, log2 n, inverse) items,
+    75% over MNT6-753 Fr and 25% over the 32-bit field, log2 n uniform over
+    the integers [10, 18], forward or inverse 50/50, in shuffled order."""
+    import random
+    rng = random.Random(seed)
+    items = []
+    for i in range(count):
+        field = _MNT6 if i < count * 3 // 4 else _FP32
+        items.append((field, rng.randint(10, 18), rng.random() < 0.5))
+    rng.shuffle(items)
+    return items
+
+
+RAGGED_INPUT_SEED = 0xBA7C4  # item k of the ragged workload draws seed RAGGED_INPUT_SEED + k
+
+
+def serving_requests(seed=0x5EED0004, count=1000):
+    """The serving mix: `count` (field, log2 n, inverse) requests, 70%
+    MNT6-753 Fr and 30% 32-bit field, log2 n uniform over the integers
+    [12, 20], forward or inverse 50/50."""
+    import random
+    rng = random.Random(seed)
+    reqs = []
+    for _ in range(count):
+        field = _MNT6 if rng.random() < 0.7 else _FP32
+        reqs.append((field, rng.randint(12, 20), rng.random() < 0.5))
+    return reqs

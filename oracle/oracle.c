@@ -190,10 +190,12 @@ void or_sub(const or_field *f, uint32_t *r, const uint32_t *a, const uint32_t *b
 }
 
 /* ciosMontgomeryMultiply, device_field_operators.h:151-187 (CIOS, Koc et
- * al.), with the correct m' and a full final subtraction. */
-void or_mul(const or_field *f, uint32_t *r, const uint32_t *a, const uint32_t *b)
+ * al.), with the correct m' and a full final subtraction.  The limb count s
+ * is a parameter of an always-inlined body so that the 24-limb and 1-limb
+ * calls compile to fully unrolled loops (the same statements). */
+static inline __attribute__((always_inline)) void cios(const or_field *f, uint32_t *r, const uint32_t *a,
+                                                       const uint32_t *b, const int s)
 {
-    const int s = f->nlimbs;
     uint32_t t[OR_MAX_LIMBS + 2];
     memset(t, 0, sizeof(t));
     for (int i = 0; i < s; ++i) {
@@ -221,6 +223,16 @@ void or_mul(const or_field *f, uint32_t *r, const uint32_t *a, const uint32_t *b
     if (t[s] || geq_n(t, f->mod, s))
         sub_n(t, t, f->mod, s);
     memcpy(r, t, sizeof(uint32_t) * s);
+}
+
+void or_mul(const or_field *f, uint32_t *r, const uint32_t *a, const uint32_t *b)
+{
+    if (f->nlimbs == 24)
+        cios(f, r, a, b, 24);
+    else if (f->nlimbs == 1)
+        cios(f, r, a, b, 1);
+    else
+        cios(f, r, a, b, f->nlimbs);
 }
 
 /* Scalar::pow (device_field_operators.h:229-258) restated as
@@ -649,6 +661,80 @@ void or_dft_direct(const or_field *f, uint32_t *out, const uint32_t *x, size_t n
         memcpy(EL(out, k, s), acc, sizeof(uint32_t) * s);
         or_mul(f, wk, wk, omega);
     }
+}
+
+/* out[i] = sum_j x[j] omega^(j ks[i]) for each of the nk indices ks[i]: the
+ * DFT's definition at chosen outputs, O(n) per index, so outputs of
+ * transforms far too large for or_dft_direct can be checked.  The j range
+ * is cut into chunks [b, e); each chunk evaluates its part by Horner's rule,
+ * sum_{j in [b,e)} x[j] w^(j - b) with w = omega^k, and is shifted by w^b;
+ * the chunks' parts are summed.  The nk Horner chains of a chunk run side by
+ * side over the same x[j]. */
+static void mul32(const or_field *f, uint32_t *r, const uint32_t *a, const uint32_t *b)
+{
+    /* one-limb Montgomery product, the s = 1 case of or_mul */
+    const uint64_t t = (uint64_t)a[0] * b[0];
+    const uint32_t m = (uint32_t)t * f->mprime;
+    const uint64_t mp = (uint64_t)m * f->mod[0];
+    /* t + m p overflows 64 bits for p > 2^31; its low words sum to 0 mod
+     * 2^32, with a carry exactly when they are non-zero */
+    const uint64_t u = (t >> 32) + (mp >> 32) + ((uint32_t)t != 0u); /* < 2p */
+    r[0] = (uint32_t)(u >= f->mod[0] ? u - f->mod[0] : u);
+}
+
+void or_dft_points(const or_field *f, uint32_t *out, const uint32_t *x, size_t n, const uint32_t *omega,
+                   const uint64_t *ks, size_t nk, int threads)
+{
+    const int s = f->nlimbs;
+    const size_t es = sizeof(uint32_t) * (size_t)s;
+    const int T = nthreads_for(threads);
+    const size_t chunks = (size_t)T * 4 < n ? (size_t)T * 4 : (n ? n : 1);
+    uint32_t *w = (uint32_t *)malloc(es * (nk ? nk : 1));
+    uint32_t *part = (uint32_t *)calloc(chunks * (nk ? nk : 1), es);
+    if (!w || !part)
+        abort();
+    for (size_t i = 0; i < nk; ++i)
+        or_pow(f, EL(w, i, s), omega, (uint64_t)(ks[i] % (n ? n : 1)));
+    const long nc = (long)chunks;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(T)
+    for (long c = 0; c < nc; ++c) {
+        const size_t b = n * (size_t)c / chunks, e = n * (size_t)(c + 1) / chunks;
+        uint32_t *acc = EL(part, (size_t)c * nk, s);
+        uint32_t t[OR_MAX_LIMBS];
+        if (s == 1) {
+            const uint32_t p = f->mod[0];
+            for (size_t j = e; j-- > b;) {
+                const uint32_t xj = x[j];
+                for (size_t i = 0; i < nk; ++i) {
+                    mul32(f, t, acc + i, w + i);
+                    const uint64_t v = (uint64_t)t[0] + xj;
+                    acc[i] = (uint32_t)(v >= p ? v - p : v);
+                }
+            }
+        } else {
+            for (size_t j = e; j-- > b;) {
+                const uint32_t *xj = EL(x, j, s);
+                for (size_t i = 0; i < nk; ++i) {
+                    uint32_t *ai = EL(acc, i, s);
+                    or_mul(f, t, ai, EL(w, i, s));
+                    or_add(f, ai, t, xj);
+                }
+            }
+        }
+        for (size_t i = 0; i < nk; ++i) {
+            uint32_t sh[OR_MAX_LIMBS];
+            or_pow(f, sh, EL(w, i, s), (uint64_t)b);
+            or_mul(f, EL(acc, i, s), EL(acc, i, s), sh);
+        }
+    }
+    for (size_t i = 0; i < nk; ++i) {
+        uint32_t sum[OR_MAX_LIMBS] = {0};
+        for (size_t c = 0; c < chunks; ++c)
+            or_add(f, sum, sum, EL(part, c * nk + i, s));
+        memcpy(EL(out, i, s), sum, es);
+    }
+    free(w);
+    free(part);
 }
 
 /* The same transform by the four-step factorisation n = n1 * n2, with

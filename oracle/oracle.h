@@ -94,6 +94,10 @@ void or_g1_msm(const or_field *f, const uint32_t *coeff_a, uint32_t *r, const ui
                const uint32_t *k, int klimbs, size_t n);
 /* Mixed-radix transforms: n = n1 * n2 (n1 odd, n2 = 2^a), omega = w2 * w1. */
 void or_dft_direct(const or_field *f, uint32_t *out, const uint32_t *x, size_t n, const uint32_t *omega);
+/* sum_j x[j] omega^(j k) at the nk indices ks (O(n) each, OpenMP over chunks
+ * of j); `threads` <= 0 means the OpenMP default. */
+void or_dft_points(const or_field *f, uint32_t *out, const uint32_t *x, size_t n, const uint32_t *omega,
+                   const uint64_t *ks, size_t nk, int threads);
 void or_mixed_fft(const or_field *f, uint32_t *a, size_t n1, size_t n2, const uint32_t *w2, const uint32_t *w1,
                   int inverse);
 void or_extended_radix2_fft(const or_field *f, uint32_t *a, size_t n, const uint32_t *omega_half,

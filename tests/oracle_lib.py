@@ -49,6 +49,8 @@ class Oracle:
                                                 ctypes.c_uint, ctypes.c_size_t, ctypes.c_size_t, _u32p, ctypes.c_int]
         L.or_inverse_fft.argtypes = [ctypes.POINTER(OrField), _u32p, ctypes.c_size_t, _u32p]
         L.or_dft_direct.argtypes = [ctypes.POINTER(OrField), _u32p, _u32p, ctypes.c_size_t, _u32p]
+        L.or_dft_points.argtypes = [ctypes.POINTER(OrField), _u32p, _u32p, ctypes.c_size_t, _u32p,
+                                    ctypes.POINTER(ctypes.c_uint64), ctypes.c_size_t, ctypes.c_int]
         L.or_mixed_fft.argtypes = [ctypes.POINTER(OrField), _u32p, ctypes.c_size_t, ctypes.c_size_t, _u32p, _u32p,
                                    ctypes.c_int]
         L.or_g1_add.argtypes = [ctypes.POINTER(OrField), _u32p, _u32p, _u32p, _u32p]
@@ -118,6 +120,15 @@ class Oracle:
         a = np.ascontiguousarray(x, np.uint32)
         out = np.zeros_like(a)
         self.lib.or_dft_direct(self.field(fid), _p(out), _p(a), a.shape[0], _p(np.ascontiguousarray(omega, np.uint32)))
+        return out
+
+    def dft_points(self, fid, x, omega, ks, threads=0):
+        """sum_j x[j] omega^(j k) for each k in ks (O(n) per index)."""
+        a = np.ascontiguousarray(x, np.uint32)
+        k = np.ascontiguousarray(np.asarray(ks, dtype=np.uint64).reshape(-1))
+        out = np.zeros((k.shape[0], NLIMBS[fid]), np.uint32)
+        self.lib.or_dft_points(self.field(fid), _p(out), _p(a), a.shape[0], _p(np.ascontiguousarray(omega, np.uint32)),
+                               k.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), k.shape[0], threads)
         return out
 
     def mixed_fft(self, fid, x, n1, w2, w1, inverse=False):
@@ -227,3 +238,51 @@ def odd_root(fid):
          if mod else int(re.search(r"P = (0x[0-9a-f]+)u", body).group(1), 16))
     rinv = pow(1 << (32 * L), -1, p)
     return M, sum(v << (32 * i) for i, v in enumerate(vals)) * rinv % p
+
+
+# ---------------------------------------------------------------- checkers
+
+def mont(fid, v):
+    """The Montgomery form (limbs) of the plain integer v."""
+    from gpusnarks_amd import synth
+    import gpusnarks_amd as g
+    p, L = g.MODULI[fid], NLIMBS[fid]
+    return synth.from_int(v * (1 << (32 * L)) % p, L)
+
+
+def primitive_root(fid, log_n):
+    """A primitive 2^log_n-th root of unity in Montgomery form, from the
+    field's smallest quadratic non-residue (5, 13, 11) with Python integers:
+    independent of the library's own root table."""
+    import gpusnarks_amd as g
+    p = g.MODULI[fid]
+    qnr = {0: 5, 1: 13, 2: 11}[fid]
+    return mont(fid, pow(qnr, (p - 1) >> log_n, p))
+
+
+def oracle_transform(orc, fid, x, inverse, omega):
+    """The oracle's forward (serial radix-2, test/fft_host.h:18-54) or
+    inverse (omega^-1 then n^-1) transform of x."""
+    if inverse:
+        return orc.inverse_fft(fid, x, omega)
+    return orc.serial_fft(fid, x, omega)
+
+
+def spot_indices(n, count, seed):
+    """k = 0, 1, n/2, n-1 and `count` further indices drawn from a seed."""
+    rng = np.random.default_rng(seed)
+    ks = [0, 1 % n, n // 2, n - 1] + [int(v) for v in rng.integers(0, n, size=count)]
+    return np.array(ks, dtype=np.uint64)
+
+
+def spot_values(orc, fid, x, omega, ks, inverse=False, threads=0):
+    """X_k for k in ks from the definition: sum_j x_j omega^(jk) forward,
+    n^-1 sum_j x_j omega^(-jk) inverse (O(n) per index)."""
+    n = x.shape[0]
+    w = orc.unary("or_inverse", fid, omega)[0] if inverse else np.ascontiguousarray(omega, np.uint32)
+    v = orc.dft_points(fid, x, w, ks, threads)
+    if inverse:
+        nm = mont(fid, n)
+        ninv = orc.unary("or_inverse", fid, nm)[0]
+        v = orc.mul(fid, v, np.broadcast_to(ninv, v.shape).copy())
+    return v

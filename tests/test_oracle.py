@@ -387,3 +387,68 @@ def test_mixed_fft_matches_direct_dft(orc, fid, n1, n2):
     got = orc.mixed_fft(fid, x, n1, mont(w2), mont(w1))
     assert np.array_equal(got, orc.dft_direct(fid, x, mont(w1 * w2 % p)))
     assert np.array_equal(orc.mixed_fft(fid, got, n1, mont(w2), mont(w1), inverse=True), x)
+
+
+@pytest.mark.parametrize("fid,log_n", [(0, 1), (0, 6), (0, 9), (1, 5), (2, 7)])
+@pytest.mark.parametrize("threads", [1, 3])
+def test_dft_points_matches_dft_golden_and_direct(orc, fid, log_n, threads):
+    """or_dft_points (X_k at chosen k, O(n) each, Horner over chunks of j)
+    equals the O(n^2) definition at every k; one and three threads cut j
+    into 4 and 12 chunks (one element per chunk at n = 2)."""
+    from tests.oracle_lib import primitive_root
+    n = 1 << log_n
+    import gpusnarks_amd as g
+    x = synth.uniform_residues(0xD0D0 + log_n, n, g.MODULI[fid], NLIMBS[fid])
+    om = primitive_root(fid, log_n)
+    want = orc.dft_direct(fid, x, om)
+    got = orc.dft_points(fid, x, om, np.arange(n), threads)
+    assert np.array_equal(got, want)
+    # indices past n wrap (omega^n = 1)
+    assert np.array_equal(orc.dft_points(fid, x, om, [n + 3 % n], threads)[0], want[3 % n])
+
+
+@pytest.mark.parametrize("name,fid", FIELDS)
+def test_dft_points_against_golden(orc, name, fid):
+    gv = golden("ntt", name)
+    n = max(int(k[1:].split("_")[0]) for k in gv.files if k.endswith("_input"))
+    x, om, X = gv["n%d_input" % n], gv["n%d_omega" % n], gv["n%d_forward" % n]
+    ks = np.array([0, 1, n // 2, n - 1, 5 % n], np.uint64)
+    assert np.array_equal(orc.dft_points(fid, x, om, ks), X[ks.astype(np.int64)])
+
+
+def test_spot_check_catches_what_a_round_trip_cannot(orc):
+    """Two wrong transforms that invert themselves exactly -- a transform
+    over w^3 instead of w, and the right outputs in a permuted order -- pass
+    inverse(forward(x)) == x but fail the spot evaluation; the true one
+    passes it (the mistakes are planted in the oracle's output, on the CPU)."""
+    from tests.oracle_lib import primitive_root, spot_indices, spot_values
+    import gpusnarks_amd as g
+    fid, log_n = 0, 12
+    n = 1 << log_n
+    x = synth.uniform_residues(0x5B07, n, g.MODULI[fid], 1)
+    om = primitive_root(fid, log_n)
+    ks = spot_indices(n, 16, 1)
+    want = spot_values(orc, fid, x, om, ks)
+    good = orc.serial_fft(fid, x, om)
+    assert np.array_equal(good[ks.astype(np.int64)], want)
+    om3 = orc.pow(fid, om[None], [3])[0]          # also a primitive 2^12-th root
+    wrong_root = orc.serial_fft(fid, x, om3)
+    assert np.array_equal(orc.inverse_fft(fid, wrong_root.copy(), om3), x)
+    assert not np.array_equal(wrong_root[ks.astype(np.int64)], want)
+    perm = np.roll(good, 1, axis=0)                 # a permutation its own inverse would undo
+    assert not np.array_equal(perm[ks.astype(np.int64)], want)
+    # inverse spot values: n^-1 sum x_j w^-jk equals the inverse transform
+    inv = orc.inverse_fft(fid, x.copy(), om)
+    assert np.array_equal(inv[ks.astype(np.int64)], spot_values(orc, fid, x, om, ks, inverse=True))
+
+
+def test_primitive_root_helper_matches_library_table():
+    """The checker's roots (Python integers from the smallest non-residue)
+    have exact order 2^k and equal the library's root table."""
+    import gpusnarks_amd as g
+    from tests.oracle_lib import primitive_root
+    for fid, k in ((0, 26), (2, 22), (1, 15)):
+        p, L = g.MODULI[fid], NLIMBS[fid]
+        w = synth.to_int(primitive_root(fid, k)) * pow(1 << (32 * L), -1, p) % p
+        assert pow(w, 1 << k, p) == 1 and pow(w, 1 << (k - 1), p) == p - 1
+        assert np.array_equal(primitive_root(fid, k), g.root_of_unity(fid, k))
