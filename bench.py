@@ -565,11 +565,23 @@ def main():
     roofline["clock_measured_ghz"] = ghz.value
     roofline["peak_at_measured_clock"] = MAD_PEAK / 1e12 * ghz.value * 1e9 / CLOCK_HZ
     roofline["frac_at_measured_clock"] = achieved / (MAD_PEAK * ghz.value * 1e9 / CLOCK_HZ)
+    # the denominator's instruction rate, measured in this run: independent
+    # v_mad_u64_u32 chains on every CU (gsn_debug_mad_rate)
+    rate, rate_ghz = ctypes.c_double(0.0), ctypes.c_double(0.0)
+    g.check(g.lib().gsn_debug_mad_rate(ctypes.byref(rate), ctypes.byref(rate_ghz)))
+    roofline["peak_probe"] = {
+        "tmad_per_s": rate.value / 1e12, "clock_ghz": rate_ghz.value,
+        "lanes_per_clk_per_cu": rate.value / (CUS * rate_ghz.value * 1e9),
+        "frac_of_probe": achieved / rate.value,
+        "kernel": "mad_rate_kernel: 8 independent v_mad_u64_u32 chains per thread, 8 waves per SIMD"}
     line_timing = step_times(field, log_n)
     traffic_file = os.path.join(ROOT, "profiles", "ntt753_n18_traffic.json")
     if os.path.exists(traffic_file):
         with open(traffic_file) as fh:
-            roofline["traffic"] = json.load(fh).get("hbm_bytes_per_launch")
+            tr = json.load(fh)
+        roofline["traffic"] = tr.get("hbm_bytes_per_launch")
+        roofline["traffic_source"] = "%s (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE of this kernel, committed " \
+                                     "summary; not measured in this run)" % tr.get("source")
     line = {
         "metric": "NTT butterflies/s, 753-bit n=2^18, batched vs plain; serving p99 latency",
         "value": value, "unit": "butterflies/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

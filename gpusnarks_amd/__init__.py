@@ -111,6 +111,7 @@ def lib():
     L.gsn_event_elapsed_ms.argtypes = [ctypes.POINTER(ctypes.c_float), vp, vp]
     L.gsn_debug_copy_u32.argtypes = [vp, vp, sz, vp]
     L.gsn_debug_clock_ghz.argtypes = [ctypes.POINTER(ctypes.c_double)]
+    L.gsn_debug_mad_rate.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     L.gsn_pointwise_mul.argtypes = [ctypes.c_int, vp, vp, vp, sz, vp]
     L.gsn_inner_product.argtypes = [ctypes.c_int, vp, vp, vp, sz, vp]
     L.gsn_vec_add.argtypes = [ctypes.c_int, vp, vp, vp, sz, vp]

@@ -100,6 +100,8 @@ struct F32 {
 struct E753 {
     uint32_t v[24];
 };
+// four u32 as a native vector (usable in any address space, unlike uint4)
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 // A twiddle of the 753-bit fields: the residue w itself (canonical, not in
 // Montgomery form) as 26 digits of 29 bits, and wq = floor(w 2^783 / p) as
 // 27 digits; padded to 224 bytes so it moves as 14 aligned quads.
@@ -446,13 +448,15 @@ struct F753 {
             r.pad[k] = 0;
         return r;
     }
-    // record i of a table of 14 planes of `stride` quads each
-    __device__ __forceinline__ static TW753 load_tw_planes(const uint4 *tab, uint32_t stride, uint32_t i)
+    // record i of a table of 14 planes of `stride` quads each (Q: uint4, or
+    // u32x4 in any address space)
+    template <class Q>
+    __device__ __forceinline__ static TW753 load_tw_planes(const Q *tab, uint32_t stride, uint32_t i)
     {
         uint32_t d[56];
 #pragma unroll
         for (int q = 0; q < 14; ++q) {
-            const uint4 v = tab[(size_t)q * stride + i];
+            const auto v = tab[(size_t)q * stride + i];
             d[4 * q] = v.x;
             d[4 * q + 1] = v.y;
             d[4 * q + 2] = v.z;

@@ -35,7 +35,7 @@ struct PassArgs {
     uint32_t col_shift;   // column table index = exponent << col_shift
     uint32_t pass_shift;  // pass table exponent = (r * k) << pass_shift
     uint32_t lo_bits;     // two-level pass table split (0: one table)
-    uint32_t tw_stride;   // 753-bit: quads per plane of the pass's step table
+    uint32_t tw_stride;   // 753-bit: quads per plane of the pass's step table (records + 1)
     uint32_t flags;
     typename F::tw_t scale;
 };
@@ -76,16 +76,21 @@ void launch_pass_multi(const PassJob<F> *d_jobs, const uint32_t *d_tile_job, uin
 
 // gsn_debug_clock_ghz's kernel (kernels_mnt6.hip)
 void launch_clock_probe(unsigned long long *spans, E753 *sink, unsigned blocks, int iters, hipStream_t stream);
+// gsn_debug_mad_rate's kernel (kernels_mnt6.hip); returns the number of
+// v_mad_u64_u32 lane operations the launch issues
+uint64_t launch_mad_rate_probe(unsigned long long *spans, unsigned long long *sink, unsigned blocks, int iters,
+                               hipStream_t stream);
 
 template <class F>
 void launch_scale(const typename F::elem_t *in, typename F::elem_t *out, typename F::tw_t s, size_t n,
                   int do_mul, hipStream_t stream);
 
 // Fills the step table of one 753-bit pass (see build_steps753_kernel):
-// nrec = Ns (2^log_r - 1) twiddle records in 14 planes of nrec quads.
-// scale0 != nullptr multiplies the step-0 records (the first Ns) by *scale0.
+// nrec = Ns (2^log_r - 1) twiddle records and the scale record *scale after
+// them, in 14 planes of nrec + 1 quads.  scale0 multiplies the step-0
+// records (the first Ns) by *scale.
 template <class F>
 void launch_build_steps753(uint4 *out, uint32_t nrec, const E753 *hi, const TW753 *lo, uint32_t h, uint32_t log_n,
-                           uint32_t log_ns, uint32_t log_r, const TW753 *scale0, hipStream_t stream);
+                           uint32_t log_ns, uint32_t log_r, const TW753 &scale, bool scale0, hipStream_t stream);
 
 }  // namespace gsn

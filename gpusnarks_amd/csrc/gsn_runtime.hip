@@ -368,13 +368,14 @@ static int build_tables_753(Plan &plan, const H &w, unsigned log_n, size_t n)
     for (size_t i = 0; i < P_; ++i) {
         const unsigned lr = (unsigned)plan.cfg[i].log_r;
         const uint32_t nrec = (uint32_t)((((size_t)1 << lr) - 1) << log_ns);
-        GSN_TRY(plan.step_tabs[i].alloc((size_t)nrec * sizeof(TW753)));
-        plan.step_nrec[i] = nrec;
+        // records, then the scale record (n^-1 for an inverse plan)
+        GSN_TRY(plan.step_tabs[i].alloc((size_t)(nrec + 1) * sizeof(TW753)));
+        plan.step_nrec[i] = nrec + 1;
         // the inverse's n^-1 rides on the last pass's step-0 records when
         // there is more than one pass (PASS_SCALE_IN)
         const bool scale0 = plan.inverse && P_ >= 2 && i == P_ - 1;
         launch_build_steps753<F>((uint4 *)plan.step_tabs[i].ptr, nrec, (const E753 *)dhi.ptr, (const TW753 *)dlo.ptr,
-                                 h, log_n, log_ns, lr, scale0 ? &plan.scale : nullptr, 0);
+                                 h, log_n, log_ns, lr, plan.scale, scale0, 0);
         GSN_TRY(hip_status(hipGetLastError(), "build_steps753 launch"));
         log_ns += lr;
     }
@@ -850,6 +851,58 @@ int gsn_debug_clock_ghz(double *ghz)
     }
     if (wall <= 0 || wall_khz <= 0)
         return set_error(GSN_ERR_DEVICE, "clock probe measured nothing");
+    *ghz = cyc / wall * (double)wall_khz * 1e-6;
+    return GSN_OK;
+}
+
+int gsn_debug_mad_rate(double *mads_per_s, double *ghz)
+{
+    if (!mads_per_s || !ghz)
+        return set_error(GSN_ERR_INVALID_ARGUMENT, "NULL argument");
+    GSN_TRY(ensure_device());
+    int dev = 0, cus = 0, wall_khz = 0;
+    GSN_TRY(hip_status(hipGetDevice(&dev), "hipGetDevice"));
+    GSN_TRY(hip_status(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "CU count"));
+    GSN_TRY(hip_status(hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, dev), "wall clock rate"));
+    const unsigned blocks = (unsigned)cus * 8;  // 8 waves per SIMD
+    const int iters = 2048;
+    DeviceBuffer spans, sink;
+    GSN_TRY(spans.alloc((size_t)blocks * 2 * sizeof(unsigned long long)));
+    GSN_TRY(sink.alloc(sizeof(unsigned long long)));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    GSN_TRY(hip_status(hipEventCreate(&e0), "hipEventCreate"));
+    int rc = hip_status(hipEventCreate(&e1), "hipEventCreate");
+    uint64_t mads = 0;
+    if (rc == GSN_OK) {
+        // the first launch lets the clock settle; the second is timed
+        launch_mad_rate_probe((unsigned long long *)spans.ptr, (unsigned long long *)sink.ptr, blocks, iters, nullptr);
+        rc = hip_status(hipEventRecord(e0, nullptr), "hipEventRecord");
+        mads = launch_mad_rate_probe((unsigned long long *)spans.ptr, (unsigned long long *)sink.ptr, blocks, iters,
+                                     nullptr);
+        if (rc == GSN_OK)
+            rc = hip_status(hipGetLastError(), "mad rate probe launch");
+        if (rc == GSN_OK)
+            rc = hip_status(hipEventRecord(e1, nullptr), "hipEventRecord");
+    }
+    float ms = 0.f;
+    if (rc == GSN_OK)
+        rc = hip_status(hipEventSynchronize(e1), "mad rate probe");
+    if (rc == GSN_OK)
+        rc = hip_status(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    (void)hipEventDestroy(e0);
+    if (e1)
+        (void)hipEventDestroy(e1);
+    GSN_TRY(rc);
+    std::vector<unsigned long long> h((size_t)blocks * 2);
+    GSN_TRY(hip_status(hipMemcpy(h.data(), spans.ptr, h.size() * sizeof(h[0]), hipMemcpyDeviceToHost), "hipMemcpy"));
+    double cyc = 0, wall = 0;
+    for (unsigned b = 0; b < blocks; ++b) {
+        cyc += (double)h[2 * b];
+        wall += (double)h[2 * b + 1];
+    }
+    if (ms <= 0.f || wall <= 0 || wall_khz <= 0)
+        return set_error(GSN_ERR_DEVICE, "mad rate probe measured nothing");
+    *mads_per_s = (double)mads / (ms * 1e-3);
     *ghz = cyc / wall * (double)wall_khz * 1e-6;
     return GSN_OK;
 }

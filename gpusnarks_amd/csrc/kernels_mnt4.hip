@@ -52,11 +52,12 @@ void launch_scale<FMnt4>(const E753 *in, E753 *out, TW753 s, size_t n, int do_mu
 
 template <>
 void launch_build_steps753<FMnt4>(uint4 *out, uint32_t nrec, const E753 *hi, const TW753 *lo, uint32_t h,
-                                 uint32_t log_n, uint32_t log_ns, uint32_t log_r, const TW753 *scale0, hipStream_t st)
+                                 uint32_t log_n, uint32_t log_ns, uint32_t log_r, const TW753 &scale, bool scale0,
+                                 hipStream_t st)
 {
-    const unsigned blocks = (nrec + 255) / 256;
+    const unsigned blocks = (nrec + 1 + 255) / 256;
     hipLaunchKernelGGL(build_steps753_kernel<FMnt4>, dim3(blocks), dim3(256), 0, st, out, nrec, hi, lo, h, log_n, log_ns,
-                       log_r, scale0 ? *scale0 : TW753{}, scale0 != nullptr);
+                       log_r, scale, (int)scale0);
 }
 
 }  // namespace gsn

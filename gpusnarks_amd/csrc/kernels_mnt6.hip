@@ -71,6 +71,46 @@ void launch_clock_probe(unsigned long long *spans, E753 *sink, unsigned blocks, 
     hipLaunchKernelGGL(clock_probe_kernel, dim3(blocks), dim3(256), 0, st, spans, sink, w, iters);
 }
 
+// Multiply-add rate probe: MAD_CHAINS independent chains per thread of
+// acc = lo32(acc) * b + acc, each one v_mad_u64_u32, so the issue rate and
+// not the latency bounds the kernel; thread 0 of each block records its
+// core-cycle and wall-clock spans as the clock probe does.
+constexpr int MAD_CHAINS = 8, MAD_UNROLL = 16;
+__global__ __launch_bounds__(256) void mad_rate_kernel(unsigned long long *spans, unsigned long long *sink, uint32_t b,
+                                                       int iters)
+{
+    uint64_t acc[MAD_CHAINS];
+#pragma unroll
+    for (int k = 0; k < MAD_CHAINS; ++k)
+        acc[k] = (uint64_t)(threadIdx.x + 1) * 0x9E3779B97F4A7C15ull + (uint64_t)(blockIdx.x * MAD_CHAINS + k);
+    const unsigned long long c0 = clock64(), r0 = wall_clock64();
+    for (int i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int u = 0; u < MAD_UNROLL; ++u)
+#pragma unroll
+            for (int k = 0; k < MAD_CHAINS; ++k)
+                acc[k] = (uint64_t)(uint32_t)acc[k] * b + acc[k];
+    }
+    const unsigned long long c1 = clock64(), r1 = wall_clock64();
+    uint64_t x = 0;
+#pragma unroll
+    for (int k = 0; k < MAD_CHAINS; ++k)
+        x ^= acc[k];
+    if (x == 0x0123456789ABCDEFull)  // never in practice; keeps the chains live
+        sink[0] = x;
+    if (threadIdx.x == 0) {
+        spans[2 * blockIdx.x] = c1 - c0;
+        spans[2 * blockIdx.x + 1] = r1 - r0;
+    }
+}
+
+uint64_t launch_mad_rate_probe(unsigned long long *spans, unsigned long long *sink, unsigned blocks, int iters,
+                               hipStream_t st)
+{
+    hipLaunchKernelGGL(mad_rate_kernel, dim3(blocks), dim3(256), 0, st, spans, sink, 0x9E3779B1u, iters);
+    return (uint64_t)blocks * 256u * (uint64_t)iters * MAD_UNROLL * MAD_CHAINS;
+}
+
 template <>
 void launch_scale<FMnt6>(const E753 *in, E753 *out, TW753 s, size_t n, int do_mul, hipStream_t st)
 {
@@ -80,11 +120,12 @@ void launch_scale<FMnt6>(const E753 *in, E753 *out, TW753 s, size_t n, int do_mu
 
 template <>
 void launch_build_steps753<FMnt6>(uint4 *out, uint32_t nrec, const E753 *hi, const TW753 *lo, uint32_t h,
-                                 uint32_t log_n, uint32_t log_ns, uint32_t log_r, const TW753 *scale0, hipStream_t st)
+                                 uint32_t log_n, uint32_t log_ns, uint32_t log_r, const TW753 &scale, bool scale0,
+                                 hipStream_t st)
 {
-    const unsigned blocks = (nrec + 255) / 256;
+    const unsigned blocks = (nrec + 1 + 255) / 256;
     hipLaunchKernelGGL(build_steps753_kernel<FMnt6>, dim3(blocks), dim3(256), 0, st, out, nrec, hi, lo, h, log_n, log_ns,
-                       log_r, scale0 ? *scale0 : TW753{}, scale0 != nullptr);
+                       log_r, scale, (int)scale0);
 }
 
 }  // namespace gsn

@@ -358,10 +358,44 @@ __device__ __forceinline__ void tile753_body(const PassArgs<F> &a, int log_r, in
     const int jj = t >> log_w;  // in [0, R/2) for active threads
     const size_t j = j0 + c;
     const uint32_t k = (uint32_t)(j & (Ns - 1));
+    // Every device-memory access goes through global (address space 1)
+    // pointers.  Through the generic pointers of a by-value argument the
+    // compiler emits flat loads, which also count against the LDS wait
+    // counter, so waiting for an LDS exchange would also wait for the next
+    // step's twiddle record, the load this loop issues early to hide.
+    typedef __attribute__((address_space(1))) const u32x4 gcq4;
+    typedef __attribute__((address_space(1))) u32x4 gq4;
+    const gcq4 *gin = (const gcq4 *)a.in;  // element i: quads 6 i .. 6 i + 5
+    gq4 *gout = (gq4 *)a.out;
+    auto gload = [&](size_t i) {
+        elem r;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const u32x4 v = gin[6 * i + q];
+            r.v[4 * q] = v.x;
+            r.v[4 * q + 1] = v.y;
+            r.v[4 * q + 2] = v.z;
+            r.v[4 * q + 3] = v.w;
+        }
+        return r;
+    };
+    auto gstore = [&](size_t i, const elem &x) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            u32x4 v;
+            v.x = x.v[4 * q];
+            v.y = x.v[4 * q + 1];
+            v.z = x.v[4 * q + 2];
+            v.w = x.v[4 * q + 3];
+            gout[6 * i + q] = v;
+        }
+    };
     // the pass's step table (build_steps753_kernel): step s, column k, kk
     // at (2^s - 1) Ns + k 2^s + kk; step log_r, past the last, reads record
-    // 0 (a harmless, branch-free load)
-    const uint4 *tab = reinterpret_cast<const uint4 *>(a.tw_col);
+    // 0 (a harmless, branch-free load); the scale record (n^-1) is the last
+    // one, read only by the passes that scale.  Reading it from the table
+    // keeps 53 uniform digits out of the scalar registers.
+    const gcq4 *tab = (const gcq4 *)a.tw_col;
     const uint32_t planes = a.tw_stride;
     auto step_tw = [&](int s, uint32_t after) {
         const uint32_t kk = (uint32_t)jj & ((1u << s) - 1u);
@@ -376,12 +410,12 @@ __device__ __forceinline__ void tile753_body(const PassArgs<F> &a, int log_r, in
         // step 0: positions 2 jj and 2 jj + 1 = rows brev(jj) and
         // brev(jj) + R/2 from global memory
         const int jr = log_r > 1 ? (int)(__builtin_bitreverse32((uint32_t)jj) >> (33 - log_r)) : 0;
-        x[1] = F::load(a.in, j + (size_t)(jr + half) * nR);
-        x[0] = F::load(a.in, j + (size_t)jr * nR);
+        x[1] = gload(j + (size_t)(jr + half) * nR);
+        x[0] = gload(j + (size_t)jr * nR);
         if (log_ns != 0) {
             x[1] = F::mul(x[1], F::load_tw_planes(tab, planes, k));
             if (a.flags & PASS_SCALE_IN)
-                x[0] = F::mul(x[0], a.scale);
+                x[0] = F::mul(x[0], F::load_tw_planes(tab, planes, planes - 1u));
         }
         w = step_tw(1, x[1].v[23]);
         const elem s0 = F::add(x[0], x[1]);
@@ -421,8 +455,9 @@ __device__ __forceinline__ void tile753_body(const PassArgs<F> &a, int log_r, in
     // x[e] holds output row jj + e*R/2 of column c
     if (on) {
         if (a.flags & PASS_SCALE) {
-            x[0] = F::mul(x[0], a.scale);
-            x[1] = F::mul(x[1], a.scale);
+            const tw sc = F::load_tw_planes(tab, planes, planes - 1u);
+            x[0] = F::mul(x[0], sc);
+            x[1] = F::mul(x[1], sc);
         }
         if (a.flags & PASS_CANON) {
             x[0] = F::canon(x[0]);
@@ -441,14 +476,13 @@ __device__ __forceinline__ void tile753_body(const PassArgs<F> &a, int log_r, in
             F::lds_store(lds, STRIDE, (jj + half) * W + c, x[1]);
         }
         __syncthreads();
-        typedef uint32_t u4v __attribute__((ext_vector_type(4)));
         const uint4 *q4 = reinterpret_cast<const uint4 *>(lds);
-        u4v *o = reinterpret_cast<u4v *>(a.out + j0 * R);
+        gq4 *o = (gq4 *)(a.out + j0 * R);
         const int nq = (R * W) * 6;
         for (int m = t; m < nq; m += (int)blockDim.x) {
             const int e = m / 6, q = m - 6 * e;
             const uint4 v = q4[q * STRIDE + lds_pad((e & (R - 1)) * W + (e >> log_r))];
-            u4v w;
+            u32x4 w;
             w.x = v.x;
             w.y = v.y;
             w.z = v.z;
@@ -458,8 +492,8 @@ __device__ __forceinline__ void tile753_body(const PassArgs<F> &a, int log_r, in
     } else if (Ns >= (size_t)W) {
         if (on) {
             const size_t base = (j0 / Ns) * Ns * R + k;
-            F::store(a.out, base + (size_t)jj * Ns, x[0]);
-            F::store(a.out, base + (size_t)(jj + half) * Ns, x[1]);
+            gstore(base + (size_t)jj * Ns, x[0]);
+            gstore(base + (size_t)(jj + half) * Ns, x[1]);
         }
     } else {
         // one contiguous run of R*W outputs: transpose through LDS
@@ -478,7 +512,7 @@ __device__ __forceinline__ void tile753_body(const PassArgs<F> &a, int log_r, in
                 const int rem = o & ((1 << (log_ns + log_r)) - 1);
                 const int r2 = rem >> log_ns;
                 const int c_lo = rem & ((1 << log_ns) - 1);
-                F::store(a.out, base + o, F::lds_load(lds, STRIDE, r2 * W + (c_hi << log_ns) + c_lo));
+                gstore(base + o, F::lds_load(lds, STRIDE, r2 * W + (c_hi << log_ns) + c_lo));
             }
         }
     }
@@ -505,9 +539,9 @@ __global__ __launch_bounds__(1 << (LOG_TILE - 1)) __attribute__((amdgpu_waves_pe
 
 // Step table of one 753-bit pass (log_ns, log_r) of a 2^log_n transform:
 // the records tile753_body reads, in the order a wave reads them, stored as
-// 14 planes of 16-byte quads (quad q of record i at q * nrec + i) so that a
-// wave's request for one quad of 64 consecutive records is one contiguous
-// kilobyte.  Step s (2^s Ns records from offset (2^s - 1) Ns) holds, at
+// 14 planes of 16-byte quads (quad q of record i at q * (nrec + 1) + i) so
+// that a wave's request for one quad of 64 consecutive records is one
+// contiguous kilobyte.  Record nrec is the pass's scale record.  Step s (2^s Ns records from offset (2^s - 1) Ns) holds, at
 // k 2^s + kk, w_{Ns 2^(s+1)}^(k + Ns kk) = w_n^e with
 // e = (k + Ns kk) << (log_n - log_ns - s - 1) < n/2; each record is
 // w_n^e = hi[e >> h] * lo[e & (2^h - 1)] reduced to [0, p).
@@ -517,8 +551,16 @@ __global__ __launch_bounds__(256) void build_steps753_kernel(uint4 *out, uint32_
                                                             TW753 scale0, int use_scale0)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nrec)
+    const uint32_t stride = nrec + 1u;
+    if (i > nrec)
         return;
+    if (i == nrec) {
+        const uint4 *q = reinterpret_cast<const uint4 *>(&scale0);
+#pragma unroll
+        for (int p = 0; p < 14; ++p)
+            out[(size_t)p * stride + i] = q[p];
+        return;
+    }
     const uint32_t s = 31u - (uint32_t)__builtin_clz((i >> log_ns) + 1u);
     const uint32_t jx = i - (((1u << s) - 1u) << log_ns);
     const uint32_t k = jx >> s, kk = jx & ((1u << s) - 1u);
@@ -530,7 +572,7 @@ __global__ __launch_bounds__(256) void build_steps753_kernel(uint4 *out, uint32_
     const uint4 *q = reinterpret_cast<const uint4 *>(&r);
 #pragma unroll
     for (int p = 0; p < 14; ++p)
-        out[(size_t)p * nrec + i] = q[p];
+        out[(size_t)p * stride + i] = q[p];
     (void)log_r;
 }
 

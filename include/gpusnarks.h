@@ -106,6 +106,11 @@ int gsn_debug_copy_u32(void *d_out, const void *d_in, size_t count, void *stream
  * counter against the constant-rate wall clock: the clock behind a measured
  * multiply rate, which under this load sits below the 2.4 GHz peak. */
 int gsn_debug_clock_ghz(double *ghz);
+/* Measures the v_mad_u64_u32 issue rate (32x32->64 multiply-adds per
+ * second, the unit of the 753-bit roofline) with independent multiply-add
+ * chains on every CU (eight waves per SIMD, HIP events around one launch of
+ * about a millisecond), and the engine clock during that launch. */
+int gsn_debug_mad_rate(double *mads_per_s, double *ghz);
 
 /* --- plain transform: the best_fft replacement ---------------------------
  * cuda/fft_kernel.cu:117-147: host vector in, transform in place,
