@@ -90,6 +90,9 @@ def lib():
                                      ctypes.POINTER(ctypes.c_size_t)]
     L.gsn_msm.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
     L.gsn_plan_execute.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    L.gsn_plan_execute_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    L.gsn_plan_workspace_bytes.restype = ctypes.c_size_t
+    L.gsn_plan_workspace_bytes.argtypes = [ctypes.c_void_p]
     L.gsn_plan_device_bytes.restype = ctypes.c_size_t
     L.gsn_plan_device_bytes.argtypes = [ctypes.c_void_p]
     L.gsn_plan_num_passes.argtypes = [ctypes.c_void_p]

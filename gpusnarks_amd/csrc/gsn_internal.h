@@ -8,6 +8,8 @@
 #include <mutex>
 #include <string>
 
+#include "gpusnarks.h"
+
 #define GSN_TRY(expr)                  \
     do {                               \
         const int gsn_st_ = (expr);    \
@@ -53,6 +55,16 @@ struct PlanBase {
     DeviceBuffer host_buf;
     virtual ~PlanBase() = default;
     virtual int execute(const void *in, void *out, hipStream_t s) = 0;
+    // Plain power-of-two plans only: execute() with the workspace in a
+    // caller's device buffer, allocating nothing; workspace_bytes() is what
+    // that buffer must hold.
+    virtual size_t workspace_bytes() const { return 0; }
+    virtual int execute_workspace(const void *in, void *out, void *ws, size_t ws_bytes, hipStream_t s)
+    {
+        (void)in, (void)out, (void)ws, (void)ws_bytes, (void)s;
+        return set_error(GSN_ERR_INVALID_ARGUMENT,
+                         "the allocation-free entry point takes plans of gsn_plan_create only");
+    }
 };
 
 // Copies of caller-owned host buffers through pinned, double-buffered

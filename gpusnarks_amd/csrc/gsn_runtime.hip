@@ -613,6 +613,54 @@ int gsn_plan_execute(gsn_plan *plan, const void *d_in, void *d_out, void *stream
     return plan->impl->execute(d_in, d_out, (hipStream_t)stream);
 }
 
+}  // extern "C"
+
+// The allocation-free entry point's workspace: one array per device, held
+// until the process ends.  Its name is a host-side stand-in, not a device
+// address: host code uses only the address hipGetSymbolAddress returns for
+// the current device (noalloc_workspace).
+static __device__ __align__(256) unsigned char noalloc_scratch[GSN_NOALLOC_WORKSPACE_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_noalloc_mutex;
+void *g_noalloc_ptr[MAX_DEVICES];  // per device; nullptr until looked up
+}  // namespace
+
+// The device address of the current device's noalloc_scratch, looked up
+// once per device with that device current.
+static int noalloc_workspace(void **out)
+{
+    int dev = 0;
+    GSN_TRY(hip_status(hipGetDevice(&dev), "hipGetDevice"));
+    if (dev < 0 || dev >= MAX_DEVICES)
+        return set_error(GSN_ERR_DEVICE, "device index out of range for the allocation-free workspace");
+    std::lock_guard<std::mutex> lk(g_noalloc_mutex);
+    if (!g_noalloc_ptr[dev]) {
+        void *p = nullptr;
+        GSN_TRY(hip_status(hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch)), "hipGetSymbolAddress"));
+        if (!p)
+            return set_error(GSN_ERR_DEVICE, "hipGetSymbolAddress returned no address");
+        g_noalloc_ptr[dev] = p;
+    }
+    *out = g_noalloc_ptr[dev];
+    return GSN_OK;
+}
+
+extern "C" {
+
+int gsn_plan_execute_noalloc(gsn_plan *plan, const void *d_in, void *d_out, void *stream)
+{
+    if (!plan || !d_in || !d_out)
+        return set_error(GSN_ERR_INVALID_ARGUMENT, "NULL plan or buffer");
+    GSN_TRY(ensure_device());
+    void *ws = nullptr;
+    GSN_TRY(noalloc_workspace(&ws));
+    return plan->impl->execute_workspace(d_in, d_out, ws, GSN_NOALLOC_WORKSPACE_BYTES, (hipStream_t)stream);
+}
+
+size_t gsn_plan_workspace_bytes(const gsn_plan *plan) { return plan ? plan->impl->workspace_bytes() : 0; }
+
 size_t gsn_plan_device_bytes(const gsn_plan *plan) { return plan ? plan->impl->device_bytes : 0; }
 
 int gsn_plan_num_passes(const gsn_plan *plan) { return plan ? plan->impl->num_passes : 0; }

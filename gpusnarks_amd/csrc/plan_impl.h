@@ -155,7 +155,13 @@ struct PlanImpl final : PlanBase {
     // alternates ws[0], ws[1], so no pass reads the buffer it writes.
     void make_jobs(const void *in, void *out, void *ws0, void *ws1, std::vector<PassJob<F>> &jobs) const
     {
-        jobs.clear();
+        jobs.resize(cfg.size());
+        make_jobs(in, out, ws0, ws1, jobs.data());
+    }
+    // The same into jobs[0 .. cfg.size()) (at most MAX_PASSES entries).
+    static constexpr int MAX_PASSES = 32;
+    void make_jobs(const void *in, void *out, void *ws0, void *ws1, PassJob<F> *jobs) const
+    {
         const int P = (int)cfg.size();
         uint32_t log_ns = 0;
         void *ws[2] = {ws0, ws1};
@@ -196,7 +202,7 @@ struct PlanImpl final : PlanBase {
             jb.log_w = (uint32_t)c.log_w;
             jb.num_tiles = (uint32_t)((((size_t)1 << log_n) >> c.log_r) >> c.log_w);
             jb.first_tile = 0;
-            jobs.push_back(jb);
+            jobs[i] = jb;
             log_ns += (uint32_t)c.log_r;
         }
     }
@@ -213,6 +219,43 @@ struct PlanImpl final : PlanBase {
         std::vector<PassJob<F>> jobs;
         make_jobs(in, out, scratch[0].ptr, scratch[1].ptr, jobs);
         for (size_t i = 0; i < jobs.size(); ++i) {
+            launch_pass<F>(jobs[i], cfg[i], s);
+            GSN_TRY(hip_status(hipGetLastError(), "ntt pass launch"));
+        }
+        return GSN_OK;
+    }
+
+    // Device bytes of the workspace an execution over a caller's buffer
+    // needs: the second buffer starts at a 256-byte boundary.
+    size_t workspace_bytes() const override
+    {
+        const size_t one = ((size_t)1 << log_n) * sizeof(elem);
+        const int nb = workspace_buffers();
+        return nb == 0 ? 0 : nb == 1 ? one : ((one + 255) & ~(size_t)255) + one;
+    }
+
+    // execute() with the workspace in the caller's device buffer `ws` of
+    // ws_bytes: no allocation of any kind (the jobs live on the stack).
+    int execute_workspace(const void *in, void *out, void *ws, size_t ws_bytes, hipStream_t s) override
+    {
+        GSN_TRY(take_injected_fault());
+        const size_t n = (size_t)1 << log_n;
+        if (cfg.empty()) {  // n == 1: canonical copy
+            launch_scale<F>((const elem *)in, (elem *)out, scale, n, 0, s);
+            return hip_status(hipGetLastError(), "scale_kernel launch");
+        }
+        if (cfg.size() > (size_t)MAX_PASSES)
+            return set_error(GSN_ERR_INVALID_ARGUMENT, "more passes than the allocation-free path holds");
+        const size_t need = workspace_bytes();
+        if (need > ws_bytes)
+            return set_error(GSN_ERR_INVALID_ARGUMENT,
+                             "the transform needs " + std::to_string(need) + " workspace bytes, more than the " +
+                                 std::to_string(ws_bytes) + " available");
+        char *w0 = (char *)ws;
+        char *w1 = w0 + ((n * sizeof(elem) + 255) & ~(size_t)255);
+        PassJob<F> jobs[MAX_PASSES];
+        make_jobs(in, out, w0, w1, jobs);
+        for (size_t i = 0; i < cfg.size(); ++i) {
             launch_pass<F>(jobs[i], cfg[i], s);
             GSN_TRY(hip_status(hipGetLastError(), "ntt pass launch"));
         }

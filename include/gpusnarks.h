@@ -185,6 +185,24 @@ int gsn_plan_create_mixed(gsn_plan **plan, int field, size_t n, int inverse);
  * `stream`.  Asynchronous; the plan's workspace is reused, so calls on one
  * plan must not overlap in time. */
 int gsn_plan_execute(gsn_plan *plan, const void *d_in, void *d_out, void *stream);
+/* gsn_plan_execute without any device allocation: no counterpart in the
+ * reference (whose best_fft allocates on every call, cuda/fft_kernel.cu:129-132).
+ * It is for callers that cannot allocate -- code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc.  The intermediate
+ * passes run in one library-owned workspace per device, a static device array
+ * of GSN_NOALLOC_WORKSPACE_BYTES (64 MiB, held until the process ends), instead
+ * of the plan's own workspace, which gsn_plan_execute allocates on first use.
+ * Plans of gsn_plan_create only (not _ext / _mixed), whose
+ * gsn_plan_workspace_bytes() is at most that limit (e.g. 753-bit fields up to
+ * 2^18, the 32-bit field up to 2^23 with the default schedules); larger ones
+ * fail with GSN_ERR_INVALID_ARGUMENT.  The workspace needs no clearing (every
+ * pass writes what the next reads).  Every call on one GPU shares its
+ * workspace: calls that overlap in time on one GPU race, from any plan,
+ * stream or thread, so serialise them (one stream, or events). */
+#define GSN_NOALLOC_WORKSPACE_BYTES ((size_t)64 << 20)
+int gsn_plan_execute_noalloc(gsn_plan *plan, const void *d_in, void *d_out, void *stream);
+/* Workspace bytes an execution of the plan needs (0 for one pass). */
+size_t gsn_plan_workspace_bytes(const gsn_plan *plan);
 /* Device bytes the plan holds (twiddles + workspace). */
 size_t gsn_plan_device_bytes(const gsn_plan *plan);
 /* Number of kernel launches (passes) per execution. */
