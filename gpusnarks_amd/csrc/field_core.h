@@ -326,6 +326,20 @@ struct F753 {
         return o;
     }
 
+    // the butterfly's sums (x0 + t, x0 + 3p - t) for t < 3p, in place (no
+    // reduction), with the three carry chains interleaved limb by limb so
+    // that no carry-in waits on the instruction right before it
+    __device__ __forceinline__ static void butterfly(E753 &x0, E753 &t)
+    {
+        unsigned c0 = 0, b1 = 0, c2 = 0;
+#pragma unroll
+        for (int i = 0; i < 24; ++i) {
+            const uint32_t a = x0.v[i], b = t.v[i];
+            const uint32_t d = __builtin_subc(P::THREE_P[i], b, b1, &b1);
+            x0.v[i] = __builtin_addc(a, b, c0, &c0);
+            t.v[i] = __builtin_addc(a, d, c2, &c2);
+        }
+    }
     // a + b (no reduction: callers keep the sum below 2^768)
     __device__ __forceinline__ static E753 add(const E753 &a, const E753 &b)
     {

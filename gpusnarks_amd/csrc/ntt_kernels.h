@@ -418,9 +418,7 @@ __device__ __forceinline__ void tile753_body(const PassArgs<F> &a, int log_r, in
                 x[0] = F::mul(x[0], F::load_tw_planes(tab, planes, planes - 1u));
         }
         w = step_tw(1, x[1].v[23]);
-        const elem s0 = F::add(x[0], x[1]);
-        x[1] = F::sub(x[0], x[1]);
-        x[0] = s0;
+        F::butterfly(x[0], x[1]);
     }
     for (int s = 1; s < log_r; ++s) {
         // x[e] is position bit s - 1 = e; thread bit tau holds position bit
@@ -447,9 +445,7 @@ __device__ __forceinline__ void tile753_body(const PassArgs<F> &a, int log_r, in
             x[1] = F::mul(F::lds_load(lds, STRIDE, active + t), w);
             w = step_tw(s + 1, x[1].v[23]);
             x[0] = F::lds_load(lds, STRIDE, t);
-            const elem s0 = F::add(x[0], x[1]);
-            x[1] = F::sub(x[0], x[1]);
-            x[0] = s0;
+            F::butterfly(x[0], x[1]);
         }
     }
     // x[e] holds output row jj + e*R/2 of column c
