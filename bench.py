@@ -522,6 +522,8 @@ def main():
     ap.add_argument("--skip-extras", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run the u32 copy kernel over this many MiB (counter calibration)")
+    ap.add_argument("--section", choices=["batched", "batched_same", "serving"], default=None,
+                    help="only run one workload of the line and print its JSON (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the box has one MI355X); "
@@ -536,6 +538,10 @@ def main():
             g.check(g.lib().gsn_debug_copy_u32(ctypes.c_void_p(dst.ptr), ctypes.c_void_p(src.ptr), nbytes // 4, None))
         g.synchronize()
         log("copied %d MiB x %d" % (args.copy_calibration_mib, args.steps))
+        return
+    if args.section:
+        fn = {"batched": bench_batched, "batched_same": bench_batched_same, "serving": bench_serving}[args.section]
+        print(json.dumps({args.section: fn()}))
         return
     field = {"mnt6": g.FIELD_MNT6753_FR, "mnt4": g.FIELD_MNT4753_FR, "fp32": g.FIELD_FP32}[args.field]
     log_n = args.log_n
