@@ -38,7 +38,8 @@ MODULI = {
 }
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
-library_path = os.path.join(_PKG, "libgpusnarks.so")
+# $GSN_LIBRARY selects another build of the library (same-box comparisons)
+library_path = os.environ.get("GSN_LIBRARY") or os.path.join(_PKG, "libgpusnarks.so")
 
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 _lib = None
