@@ -622,6 +622,18 @@ def main():
             "actual_pass_bytes_GBps": 8 * (1 << 26) * p32 / (ms32 * 1e-3) / 1e9,
             "per_transform": step_times(g.FIELD_FP32, 26),
         }
+        valu_file = os.path.join(ROOT, "profiles", "ntt_fp32_n26_valu.json")
+        if os.path.exists(valu_file):
+            # the time the passes would take with the VALU issuing every cycle
+            # (their counted instructions at 2 or 4 cycles each, committed
+            # counter summary): the ceiling this butterfly leaves for HBM use
+            with open(valu_file) as fh:
+                vb = json.load(fh)["valu_bound_ms"]
+            line["fp32_n26"]["valu_ceiling"] = {
+                "valu_bound_ms": vb, "valu_ceiling_frac": vb / ms32,
+                "max_frac_of_8TBps_at_valu_bound": bytes_alg / (vb * 1e-3) / HBM_PEAK,
+                "source": "profiles/ntt_fp32_n26_counters.txt (SQ_INSTS_VALU per pass x the kernels' cycles per "
+                          "instruction; not measured in this run)"}
         log("inner product")
         line["inner_product"] = bench_inner_product()
         log("end to end (host buffers)")

@@ -104,11 +104,14 @@ struct E753 {
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 // A twiddle of the 753-bit fields: the residue w itself (canonical, not in
 // Montgomery form) as 26 digits of 29 bits, and wq = floor(w 2^783 / p) as
-// 27 digits; padded to 224 bytes so it moves as 14 aligned quads.
+// 27 digits; padded to 224 bytes so it moves as 14 aligned quads, w in
+// quads 0..6 and wq in quads 7..13 (the product uses them at different
+// times, so each half can be loaded on its own).
 struct TW753 {
     uint32_t w[26];
+    uint32_t pad0[2];
     uint32_t wq[27];
-    uint32_t pad[3];
+    uint32_t pad1;
 };
 
 // Digits [29i, 29i + 29) of a value held in 24 u32 limbs, 27 digits.
@@ -285,27 +288,29 @@ struct F753 {
         return d;
     }
 
-    // y * w mod p as a value in [0, 3p), for any y < 2^768
-    __device__ __forceinline__ static E753 mul(const E753 &y, const TW753 &t)
+    // y * w mod p as a value in [0, 3p), for any y < 2^768, in two halves
+    // (so that a caller can issue loads between them): mul_q forms the
+    // quotient estimate from the digits a of y and wq, mul_r the remainder.
+    // q = floor(a * wq / 2^783): columns 25..52, digits from column 27
+    __device__ __forceinline__ static void mul_q(uint32_t q[27], const uint32_t a[27], const uint32_t wq[27])
     {
-        uint32_t a[27];
-        to29(a, y.v);
-        // q = floor(a * wq / 2^783): columns 25..52, digits from column 27
-        uint32_t q[27];
         uint64_t acc = 0;
 #pragma unroll
         for (int c = 25; c <= 52; ++c) {
 #pragma unroll
             for (int i = (c > 26 ? c - 26 : 0); i <= (c < 26 ? c : 26); ++i)
-                acc += (uint64_t)a[i] * t.wq[c - i];
+                acc += (uint64_t)a[i] * wq[c - i];
             if (c >= 27)
                 q[c - 27] = (uint32_t)acc & M29;
             acc >>= 29;
         }
         q[26] = (uint32_t)acc;
-        // a*w - q*p modulo 2^783: columns 0..26.  The digit -p_0 goes
-        // through an SGPR the compiler cannot see into: for p_0 = 1 it would
-        // otherwise turn q_c * -1 into a two-instruction 64-bit subtraction.
+    }
+    // a*w - q*p modulo 2^783: columns 0..26.  The digit -p_0 goes through
+    // an SGPR the compiler cannot see into: for p_0 = 1 it would otherwise
+    // turn q_c * -1 into a two-instruction 64-bit subtraction.
+    __device__ __forceinline__ static E753 mul_r(const uint32_t a[27], const uint32_t w[26], const uint32_t q[27])
+    {
         int32_t neg_p0;
         asm("s_mov_b32 %0, %1" : "=s"(neg_p0) : "i"(-(int32_t)P::MOD29[0]));
         uint32_t r[27];
@@ -314,7 +319,7 @@ struct F753 {
         for (int c = 0; c <= 26; ++c) {
 #pragma unroll
             for (int i = (c > 25 ? c - 25 : 0); i <= (c < 26 ? c : 26); ++i)
-                s += (int64_t)((uint64_t)a[i] * t.w[c - i]);
+                s += (int64_t)((uint64_t)a[i] * w[c - i]);
 #pragma unroll
             for (int i = (c > 25 ? c - 25 : 0); i <= (c < 26 ? c : 26); ++i)
                 s += (int64_t)(int32_t)q[i] * (int64_t)(c - i == 0 ? neg_p0 : -(int32_t)P::MOD29[c - i]);
@@ -324,6 +329,13 @@ struct F753 {
         E753 o;
         from29(o.v, r);
         return o;
+    }
+    __device__ __forceinline__ static E753 mul(const E753 &y, const TW753 &t)
+    {
+        uint32_t a[27], q[27];
+        to29(a, y.v);
+        mul_q(q, a, t.wq);
+        return mul_r(a, t.w, q);
     }
 
     // the butterfly's sums (x0 + t, x0 + 3p - t) for t < 3p, in place (no
@@ -413,7 +425,7 @@ struct F753 {
         for (int i = 0; i < 26; ++i)
             o.w[i] = d[i];
         shoup_wq<P>(o.wq, rem.v);
-        o.pad[0] = o.pad[1] = o.pad[2] = 0;
+        o.pad0[0] = o.pad0[1] = o.pad1 = 0;
         return o;
     }
 
@@ -456,10 +468,8 @@ struct F753 {
             r.w[k] = d[k];
 #pragma unroll
         for (int k = 0; k < 27; ++k)
-            r.wq[k] = d[26 + k];
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-            r.pad[k] = 0;
+            r.wq[k] = d[28 + k];
+        r.pad0[0] = r.pad0[1] = r.pad1 = 0;
         return r;
     }
     // record i of a table of 14 planes of `stride` quads each (Q: uint4, or
@@ -482,10 +492,8 @@ struct F753 {
             r.w[k] = d[k];
 #pragma unroll
         for (int k = 0; k < 27; ++k)
-            r.wq[k] = d[26 + k];
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-            r.pad[k] = 0;
+            r.wq[k] = d[28 + k];
+        r.pad0[0] = r.pad0[1] = r.pad1 = 0;
         return r;
     }
     // LDS: six planes of 16-byte quads; limbs 4q..4q+3 of element i sit in

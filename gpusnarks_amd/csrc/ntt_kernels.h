@@ -397,11 +397,48 @@ __device__ __forceinline__ void tile753_body(const PassArgs<F> &a, int log_r, in
     // keeps 53 uniform digits out of the scalar registers.
     const gcq4 *tab = (const gcq4 *)a.tw_col;
     const uint32_t planes = a.tw_stride;
+    // Record i: the wq half (planes 7..13) first, since the product's
+    // quotient half needs only it and the w half (planes 0..6) may still be
+    // in flight then; the quads that end in padding are loaded as 2 or 3
+    // words, so that no load writes a register that nothing reads (the
+    // compiler would reuse such a register and wait for the load first).
+    typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+    typedef __attribute__((address_space(1))) const u32x2 gcq2;
+    typedef __attribute__((address_space(1))) const uint32_t gcq1;
+    auto load_record = [&](uint32_t i) {
+        tw r;
+#pragma unroll
+        for (int q = 7; q < 13; ++q) {
+            const u32x4 v = tab[(size_t)q * planes + i];
+            r.wq[4 * q - 28] = v.x;
+            r.wq[4 * q - 27] = v.y;
+            r.wq[4 * q - 26] = v.z;
+            r.wq[4 * q - 25] = v.w;
+        }
+        const gcq2 *p13 = (const gcq2 *)(tab + (size_t)13 * planes + i);
+        const u32x2 v13 = p13[0];
+        r.wq[24] = v13.x;
+        r.wq[25] = v13.y;
+        r.wq[26] = ((const gcq1 *)p13)[2];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const u32x4 v = tab[(size_t)q * planes + i];
+            r.w[4 * q] = v.x;
+            r.w[4 * q + 1] = v.y;
+            r.w[4 * q + 2] = v.z;
+            r.w[4 * q + 3] = v.w;
+        }
+        const u32x2 v6 = ((const gcq2 *)(tab + (size_t)6 * planes + i))[0];
+        r.w[24] = v6.x;
+        r.w[25] = v6.y;
+        r.pad0[0] = r.pad0[1] = r.pad1 = 0;
+        return r;
+    };
     auto step_tw = [&](int s, uint32_t after) {
         const uint32_t kk = (uint32_t)jj & ((1u << s) - 1u);
         uint32_t idx = s < log_r ? (((1u << s) - 1u) << log_ns) + (k << s) + kk : 0u;
         asm volatile("" : "+v"(idx) : "v"(after));
-        return F::load_tw_planes(tab, planes, idx);
+        return load_record(idx);
     };
 
     elem x[2];
@@ -413,9 +450,9 @@ __device__ __forceinline__ void tile753_body(const PassArgs<F> &a, int log_r, in
         x[1] = gload(j + (size_t)(jr + half) * nR);
         x[0] = gload(j + (size_t)jr * nR);
         if (log_ns != 0) {
-            x[1] = F::mul(x[1], F::load_tw_planes(tab, planes, k));
+            x[1] = F::mul(x[1], load_record(k));
             if (a.flags & PASS_SCALE_IN)
-                x[0] = F::mul(x[0], F::load_tw_planes(tab, planes, planes - 1u));
+                x[0] = F::mul(x[0], load_record(planes - 1u));
         }
         w = step_tw(1, x[1].v[23]);
         F::butterfly(x[0], x[1]);
@@ -451,7 +488,7 @@ __device__ __forceinline__ void tile753_body(const PassArgs<F> &a, int log_r, in
     // x[e] holds output row jj + e*R/2 of column c
     if (on) {
         if (a.flags & PASS_SCALE) {
-            const tw sc = F::load_tw_planes(tab, planes, planes - 1u);
+            const tw sc = load_record(planes - 1u);
             x[0] = F::mul(x[0], sc);
             x[1] = F::mul(x[1], sc);
         }
