@@ -120,11 +120,22 @@ def tune(fields, sizes=None, budget_s=60.0, log=print):
     return results
 
 
-def write_table(results, path):
-    lines = ["# field log_n  log_r,log_w,log_e per pass  (written by gpusnarks_amd.autotune)"]
+def batch_lines(path):
+    """The 'batch ...' lines of an existing table (schedules of transforms
+    fused into batches, which this tuner, timing transforms alone, keeps)."""
+    if not os.path.exists(path):
+        return []
+    with open(path) as fh:
+        return [l.rstrip("\n") for l in fh if l.split("#", 1)[0].split()[:1] == ["batch"]]
+
+
+def write_table(results, path, keep_from=None):
+    lines = ["# field log_n  log_r,log_w,log_e per pass  (written by gpusnarks_amd.autotune)",
+             "# 'batch field log_n ...': the schedule of transforms fused into batches, where it differs"]
     for (field, log_n), (sched, ms, ms_default) in sorted(results.items()):
         lines.append("%d %d %s  # %.4f ms (default %.4f ms)"
                      % (field, log_n, " ".join("%d,%d,%d" % s for s in sched), ms, ms_default))
+    lines += batch_lines(keep_from or path)
     with open(path, "w") as fh:
         fh.write("\n".join(lines) + "\n")
 
@@ -138,7 +149,7 @@ def main():
     args = ap.parse_args()
     sizes = [int(s) for s in args.sizes.split(",") if s] or None
     res = tune(args.fields.split(","), sizes, args.budget_s, log=lambda *a: print(*a, flush=True))
-    write_table(res, args.out)
+    write_table(res, args.out, keep_from=os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.txt"))
     print("wrote", args.out, flush=True)
 
 

@@ -134,7 +134,7 @@ size_t batch_workspace_bytes(const gsn_batch_item *items, size_t count)
 {
     size_t bytes = 0;
     for (size_t i = 0; i < count; ++i) {
-        const std::vector<PassCfg> v = choose_passes(items[i].field, items[i].log_n);
+        const std::vector<PassCfg> v = choose_passes(items[i].field, items[i].log_n, true);
         const int nbuf = v.size() >= 3 ? 2 : (v.size() == 2 ? 1 : 0);
         bytes += (size_t)nbuf * ((size_t)1 << items[i].log_n) * element_bytes(items[i].field);
         bytes = (bytes + 255) & ~(size_t)255;
@@ -153,7 +153,7 @@ int build_batch(BatchImpl &b, const gsn_batch_item *items, size_t count, int use
         const gsn_batch_item &it = items[i];
         if (!it.d_in || !it.d_out)
             return set_error(GSN_ERR_INVALID_ARGUMENT, "batch item " + std::to_string(i) + " has a NULL buffer");
-        GSN_TRY(cached_plan(&plans[i], it.field, it.log_n, it.omega, it.inverse));
+        GSN_TRY(cached_plan(&plans[i], it.field, it.log_n, it.omega, it.inverse, true));
         const size_t eb = element_bytes(it.field);
         const size_t n = (size_t)1 << it.log_n;
         const int P = plans[i]->num_passes;

@@ -50,6 +50,7 @@ struct PlanBase {
     unsigned log_n = 0;
     int inverse = 0;
     int num_passes = 0;
+    bool batch_mode = false;  // schedule of the batch table (fused launches)
     size_t device_bytes = 0;
     std::mutex host_mutex;  // serialises the host entry point's buffer
     DeviceBuffer host_buf;
@@ -74,10 +75,12 @@ struct PlanBase {
 int staged_h2d(void *dev, const void *host, size_t bytes, hipStream_t st);
 int staged_d2h(void *host, const void *dev, size_t bytes, hipStream_t st);
 
-int create_plan(PlanBase **out, int field, unsigned log_n, const void *omega, int inverse);
+// batch = true: a plan whose passes run fused with other transforms' (the
+// batch table's schedule)
+int create_plan(PlanBase **out, int field, unsigned log_n, const void *omega, int inverse, bool batch = false);
 void clear_schedules();
 void clear_plan_cache();
-int cached_plan(PlanBase **out, int field, unsigned log_n, const void *omega, int inverse);
+int cached_plan(PlanBase **out, int field, unsigned log_n, const void *omega, int inverse, bool batch = false);
 
 }  // namespace gsn
 

@@ -115,6 +115,7 @@ std::vector<PassCfg> default_passes(int field, unsigned L)
 namespace {
 std::mutex g_tune_mutex;
 std::map<std::pair<int, unsigned>, std::vector<PassCfg>> g_tuned;
+std::map<std::pair<int, unsigned>, std::vector<PassCfg>> g_tuned_batch;  // overrides for fused launches
 }  // namespace
 
 int load_schedules(const char *path);
@@ -138,11 +139,16 @@ static void load_default_schedules()
     }
 }
 
-std::vector<PassCfg> choose_passes(int field, unsigned log_n)
+std::vector<PassCfg> choose_passes(int field, unsigned log_n, bool batch)
 {
     static std::once_flag once;
     std::call_once(once, load_default_schedules);
     std::lock_guard<std::mutex> lk(g_tune_mutex);
+    if (batch) {
+        auto ib = g_tuned_batch.find({field, log_n});
+        if (ib != g_tuned_batch.end())
+            return ib->second;
+    }
     auto it = g_tuned.find({field, log_n});
     if (it != g_tuned.end())
         return it->second;
@@ -173,11 +179,11 @@ static int validate_schedule(int field, unsigned log_n, const std::vector<PassCf
     return GSN_OK;
 }
 
-int set_schedule(int field, unsigned log_n, const std::vector<PassCfg> &v)
+int set_schedule(int field, unsigned log_n, const std::vector<PassCfg> &v, bool batch = false)
 {
     GSN_TRY(validate_schedule(field, log_n, v));
     std::lock_guard<std::mutex> lk(g_tune_mutex);
-    g_tuned[{field, log_n}] = v;
+    (batch ? g_tuned_batch : g_tuned)[{field, log_n}] = v;
     return GSN_OK;
 }
 
@@ -185,9 +191,11 @@ void clear_schedules()
 {
     std::lock_guard<std::mutex> lk(g_tune_mutex);
     g_tuned.clear();
+    g_tuned_batch.clear();
 }
 
-// "field log_n r,w,e r,w,e ..." per line; '#' starts a comment
+// "field log_n r,w,e r,w,e ..." per line, "batch field log_n ..." for the
+// schedule of transforms fused into batches; '#' starts a comment
 int load_schedules(const char *path)
 {
     std::ifstream f(path);
@@ -201,6 +209,14 @@ int load_schedules(const char *path)
         if (hash != std::string::npos)
             line.resize(hash);
         std::istringstream is(line);
+        const bool batch = line.find("batch") != std::string::npos;
+        if (batch) {
+            std::string word;
+            is >> word;
+            if (word != "batch")
+                return set_error(GSN_ERR_INVALID_ARGUMENT,
+                                 std::string(path) + ":" + std::to_string(lineno) + ": 'batch' must come first");
+        }
         int field;
         unsigned log_n;
         if (!(is >> field >> log_n))
@@ -214,7 +230,7 @@ int load_schedules(const char *path)
                                                                ": bad pass shape '" + tok + "'");
             v.push_back(c);
         }
-        GSN_TRY(set_schedule(field, log_n, v));
+        GSN_TRY(set_schedule(field, log_n, v, batch));
     }
     return GSN_OK;
 }
@@ -394,7 +410,7 @@ int PlanImpl<GSN_FIELD_MNT6753_FR>::build_tables(const fields::ScalarMNT6 &w, si
 }
 
 template <int FID>
-static int create_plan_t(PlanBase **out, unsigned log_n, const void *omega, int inverse)
+static int create_plan_t(PlanBase **out, unsigned log_n, const void *omega, int inverse, bool batch)
 {
     using Info = FieldInfo<FID>;
     using H = typename Info::Host;
@@ -415,19 +431,20 @@ static int create_plan_t(PlanBase **out, unsigned log_n, const void *omega, int 
     p->field = FID;
     p->log_n = log_n;
     p->inverse = inverse ? 1 : 0;
+    p->batch_mode = batch;
     GSN_TRY(p->build(w));
     *out = p.release();
     return GSN_OK;
 }
 
-int create_plan(PlanBase **out, int field, unsigned log_n, const void *omega, int inverse)
+int create_plan(PlanBase **out, int field, unsigned log_n, const void *omega, int inverse, bool batch)
 {
     if (!out)
         return set_error(GSN_ERR_INVALID_ARGUMENT, "plan pointer is NULL");
     switch (field) {
-    case GSN_FIELD_FP32: return create_plan_t<GSN_FIELD_FP32>(out, log_n, omega, inverse);
-    case GSN_FIELD_MNT4753_FR: return create_plan_t<GSN_FIELD_MNT4753_FR>(out, log_n, omega, inverse);
-    case GSN_FIELD_MNT6753_FR: return create_plan_t<GSN_FIELD_MNT6753_FR>(out, log_n, omega, inverse);
+    case GSN_FIELD_FP32: return create_plan_t<GSN_FIELD_FP32>(out, log_n, omega, inverse, batch);
+    case GSN_FIELD_MNT4753_FR: return create_plan_t<GSN_FIELD_MNT4753_FR>(out, log_n, omega, inverse, batch);
+    case GSN_FIELD_MNT6753_FR: return create_plan_t<GSN_FIELD_MNT6753_FR>(out, log_n, omega, inverse, batch);
     default: return set_error(GSN_ERR_INVALID_ARGUMENT, "unknown field id " + std::to_string(field));
     }
 }
@@ -479,12 +496,13 @@ void clear_plan_cache()
     g_plan_cache.clear();
 }
 
-int cached_plan(PlanBase **out, int field, unsigned log_n, const void *omega, int inverse)
+int cached_plan(PlanBase **out, int field, unsigned log_n, const void *omega, int inverse, bool batch)
 {
     std::string key(16, '\0');
     std::memcpy(&key[0], &field, 4);
     std::memcpy(&key[4], &log_n, 4);
     std::memcpy(&key[8], &inverse, 4);
+    key[12] = batch ? 1 : 0;
     if (omega)
         key.append((const char *)omega, element_bytes(field));
     std::lock_guard<std::mutex> lk(g_cache_mutex);
@@ -494,7 +512,7 @@ int cached_plan(PlanBase **out, int field, unsigned log_n, const void *omega, in
         return GSN_OK;
     }
     PlanBase *p = nullptr;
-    GSN_TRY(create_plan(&p, field, log_n, omega, inverse));
+    GSN_TRY(create_plan(&p, field, log_n, omega, inverse, batch));
     g_plan_cache[key].reset(p);
     *out = p;
     return GSN_OK;

@@ -154,7 +154,7 @@ int enqueue_single_t(PlanBase *base, const gsn_batch_item &it, Wave *w, hipStrea
 int enqueue_single(const gsn_batch_item &it, Wave *w, hipStream_t s)
 {
     PlanBase *p = nullptr;
-    GSN_TRY(cached_plan(&p, it.field, it.log_n, it.omega, it.inverse));
+    GSN_TRY(cached_plan(&p, it.field, it.log_n, it.omega, it.inverse, true));
     switch (it.field) {
     case GSN_FIELD_FP32: return enqueue_single_t<GSN_FIELD_FP32>(p, it, w, s);
     case GSN_FIELD_MNT4753_FR: return enqueue_single_t<GSN_FIELD_MNT4753_FR>(p, it, w, s);
@@ -167,7 +167,7 @@ int enqueue_single(const gsn_batch_item &it, Wave *w, hipStream_t s)
 size_t request_bytes(int field, unsigned log_n, bool device)
 {
     const size_t n = (size_t)1 << log_n, eb = element_bytes(field);
-    const std::vector<PassCfg> v = choose_passes(field, log_n);
+    const std::vector<PassCfg> v = choose_passes(field, log_n, true);
     const int nbuf = v.size() >= 3 ? 2 : (v.size() == 2 ? 1 : 0);
     size_t bytes = device ? 0 : align_up(n * eb);
     bytes += align_up((size_t)nbuf * n * eb);

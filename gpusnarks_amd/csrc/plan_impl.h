@@ -116,7 +116,7 @@ struct PlanImpl final : PlanBase {
     {
         const H w = inverse ? omega_fwd.mul_inv() : omega_fwd;
         const size_t n = (size_t)1 << log_n;
-        cfg = choose_passes(FID, log_n);
+        cfg = choose_passes(FID, log_n, batch_mode);
         num_passes = (int)cfg.size();
         for (const PassCfg &c : cfg)
             if (!pass_supported<F>(c))

@@ -143,3 +143,26 @@ def test_transform_without_device_is_an_error():
     with pytest.raises(g.GsnError) as e:
         g.best_fft(x, None, g.FIELD_FP32)
     assert e.value.status == 5  # GSN_ERR_NO_DEVICE
+
+
+def test_tuning_table_alone_and_batch_lines(tmp_path):
+    """tuning.txt holds the schedule of a transform that runs alone and,
+    on 'batch' lines, the one of transforms fused into batches: loading a
+    table with both keeps the first for plain plans; a misplaced 'batch'
+    word is an error."""
+    shipped = os.path.join(os.path.dirname(g.library_path), "tuning.txt")
+    try:
+        g.tuning_clear()
+        g.tuning_load(shipped)
+        assert g.tuning_get(g.FIELD_MNT6753_FR, 18) == [(9, 1, 1), (9, 1, 1)]
+        t = tmp_path / "t.txt"
+        t.write_text("2 14 7,0,1 7,0,1\nbatch 2 14 6,1,1 8,0,1  # fused\n")
+        g.tuning_load(str(t))
+        assert g.tuning_get(g.FIELD_MNT6753_FR, 14) == [(7, 0, 1), (7, 0, 1)]
+        bad = tmp_path / "bad.txt"
+        bad.write_text("2 14 batch 7,0,1 7,0,1\n")
+        with pytest.raises(g.GsnError):
+            g.tuning_load(str(bad))
+    finally:
+        g.tuning_clear()
+        g.tuning_load(shipped)
