@@ -344,7 +344,7 @@ def bench_configs(threads, skip_cpu):
     return out
 
 
-def bench_batched(reps=5):
+def bench_batched(reps=10, warmup=3):
     import gpusnarks_amd as g
     from gpusnarks_amd import synth
     items = synth.ragged_workload()
@@ -365,10 +365,9 @@ def bench_batched(reps=5):
             plans[(field, log_n, inv)].execute(di.ptr, do.ptr, s.handle)
 
     batch = g.Batch([(f, l, inv, di.ptr, do.ptr) for (f, l, inv), (di, do, _) in zip(items, bufs)], use_graph=True)
+    run_batch = lambda: batch.launch(s.handle)
 
     def timed(fn):
-        fn()
-        s.synchronize()
         e0, e1 = g.Event(), g.Event()
         e0.record(s.handle)
         for _ in range(reps):
@@ -377,9 +376,17 @@ def bench_batched(reps=5):
         e1.synchronize()
         return g.Event.elapsed_ms(e0, e1) / reps
 
+    # both forms warmed up (the engine clock settles over the first
+    # milliseconds of load), then timed alternately, twice each
+    for _ in range(warmup):
+        run_plain()
+        run_batch()
+    s.synchronize()
     ms_plain = timed(run_plain)
     outs_plain = [do.download(x.shape) for _, do, x in bufs]
-    ms_batch = timed(lambda: batch.launch(s.handle))
+    ms_batch = timed(run_batch)
+    ms_plain = (ms_plain + timed(run_plain)) / 2
+    ms_batch = (ms_batch + timed(run_batch)) / 2
     same = all(np.array_equal(do.download(x.shape), ref) for (_, do, x), ref in zip(bufs, outs_plain))
     return {
         "transforms": len(items), "mix": "192 x MNT6-753 Fr + 64 x Fp32, log2 n uniform in [10,18], fwd/inv 50/50",
@@ -392,7 +399,7 @@ def bench_batched(reps=5):
     }
 
 
-def bench_batched_same(log_n=18, count=16, reps=5):
+def bench_batched_same(log_n=18, count=16, reps=10, warmup=3):
     """`count` MNT6-753 Fr transforms of one size as one fused schedule
     (hipGraph), against looping the plain plan over the same operands."""
     import gpusnarks_amd as g
@@ -406,8 +413,6 @@ def bench_batched_same(log_n=18, count=16, reps=5):
     batch = g.Batch([(field, log_n, False, di.ptr, do.ptr) for di, do, _ in bufs], use_graph=True)
 
     def timed(fn):
-        fn()
-        s.synchronize()
         e0, e1 = g.Event(), g.Event()
         e0.record(s.handle)
         for _ in range(reps):
@@ -420,9 +425,16 @@ def bench_batched_same(log_n=18, count=16, reps=5):
         for di, do, _ in bufs:
             plan.execute(di.ptr, do.ptr, s.handle)
 
+    run_batch = lambda: batch.launch(s.handle)
+    for _ in range(warmup):
+        run_plain()
+        run_batch()
+    s.synchronize()
     ms_plain = timed(run_plain)
     ref = [do.download(x.shape) for _, do, x in bufs]
-    ms_batch = timed(lambda: batch.launch(s.handle))
+    ms_batch = timed(run_batch)
+    ms_plain = (ms_plain + timed(run_plain)) / 2
+    ms_batch = (ms_batch + timed(run_batch)) / 2
     same = all(np.array_equal(do.download(x.shape), r) for (_, do, x), r in zip(bufs, ref))
     total = count * bfly(log_n)
     return {"transforms": count, "log2_n": log_n, "field": "MNT6-753 Fr",
