@@ -45,6 +45,14 @@ HBM_PEAK = 8.0e12
 # 2 * 24^2 + 24 = 1176 32x32->64 multiply-adds (add/sub not counted)
 MADS_PER_BFLY_753 = 1176
 REF_DERIVED_BFLY_S = 1.10e5  # reference README.md:46, 2^18 in 21.42 s, (n/2) log2 n / t
+# untimed GPU time before each timed region of plain transforms (engine
+# clock ramp under sustained load); --settle-ms overrides it
+SETTLE_MS = 100.0
+
+
+def set_settle_ms(v):
+    global SETTLE_MS
+    SETTLE_MS = v
 
 
 def log(*a):
@@ -61,9 +69,16 @@ def inputs(field, log_n, seed):
     return synth.uniform_residues(seed, 1 << log_n, g.MODULI[field], g.NLIMBS[field])
 
 
-def time_plan(field, log_n, steps, warmup, inverse=False, want_output=False):
-    """Device-resident transform, timed with HIP events on its own stream."""
+def time_plan(field, log_n, steps, warmup, inverse=False, want_output=False, settle_ms=None, info=None):
+    """Device-resident transform, timed with HIP events on its own stream:
+    `warmup` untimed transforms, then untimed transforms until `settle_ms` of
+    GPU time has passed (the engine clock rises over the first ~25 ms of
+    sustained load: MNT6-753 Fr 2^18 runs at 0.150 ms per transform after 5
+    warm-up transforms and at 0.130 ms after 200), then `steps` timed back to
+    back.  With `info`, the same `steps` are also timed right after the
+    `warmup` transforms, before the settling, and reported there."""
     import gpusnarks_amd as g
+    settle_ms = SETTLE_MS if settle_ms is None else settle_ms
     x = inputs(field, log_n, 0x5EED0000 + log_n)
     plan = g.Plan(field, log_n, inverse=inverse)
     d_in = g.DeviceArray.from_numpy(x)
@@ -73,12 +88,24 @@ def time_plan(field, log_n, steps, warmup, inverse=False, want_output=False):
         plan.execute(d_in.ptr, d_out.ptr, s.handle)
     s.synchronize()
     e0, e1 = g.Event(), g.Event()
-    e0.record(s.handle)
-    for _ in range(steps):
-        plan.execute(d_in.ptr, d_out.ptr, s.handle)
-    e1.record(s.handle)
-    e1.synchronize()
-    ms = g.Event.elapsed_ms(e0, e1)
+
+    def timed(k):
+        e0.record(s.handle)
+        for _ in range(k):
+            plan.execute(d_in.ptr, d_out.ptr, s.handle)
+        e1.record(s.handle)
+        e1.synchronize()
+        return g.Event.elapsed_ms(e0, e1)
+
+    cold = timed(steps) if info is not None else None
+    spent, extra = 0.0, 0
+    while spent < settle_ms:
+        spent += timed(max(steps, 8))
+        extra += max(steps, 8)
+    ms = timed(steps)
+    if info is not None:
+        info.update({"warmup_steps": warmup, "settle_steps": extra + steps, "settle_gpu_ms": spent + cold,
+                     "ms_per_step_after_warmup_only": cold / steps})
     # spot check against a host round trip of the same input
     out = d_out.download(x.shape)
     ref = g.best_fft(x, None, field, inverse=inverse)
@@ -534,9 +561,12 @@ def main():
     ap.add_argument("--skip-extras", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run the u32 copy kernel over this many MiB (counter calibration)")
+    ap.add_argument("--settle-ms", type=float, default=SETTLE_MS,
+                    help="untimed GPU time after the warm-up transforms, before each timed region")
     ap.add_argument("--section", choices=["batched", "batched_same", "serving"], default=None,
                     help="only run one workload of the line and print its JSON (profiling runs)")
     args = ap.parse_args()
+    set_settle_ms(args.settle_ms)
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the box has one MI355X); "
               "run with --gpus 1", file=sys.stderr)
@@ -563,7 +593,8 @@ def main():
         print(json.dumps({"field": args.field, "log2_n": log_n, "ms": ms, "passes": passes,
                           "butterflies_per_s": bfly(log_n) / (ms * 1e-3)}))
         return
-    ms, passes, gpu_out = time_plan(field, log_n, args.steps, args.warmup, want_output=True)
+    settle = {}
+    ms, passes, gpu_out = time_plan(field, log_n, args.steps, args.warmup, want_output=True, info=settle)
     value = bfly(log_n) / (ms * 1e-3)
     ms_launch = ms / passes
     mads_per_launch = bfly(log_n) / passes * MADS_PER_BFLY_753
@@ -611,6 +642,9 @@ def main():
                    "field": "MNT6-753 Fr", "log2_n": log_n, "elements_per_s": (1 << log_n) / (ms * 1e-3)},
         "roofline": roofline,
         "per_transform": line_timing,
+        "clock_settle": dict(settle, note="after the `warmup` transforms, untimed transforms run until %g ms of "
+                             "GPU time has passed; ms_per_step_after_warmup_only times the same steps before "
+                             "that" % SETTLE_MS),
     }
     if not args.skip_cpu:
         threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
@@ -618,7 +652,7 @@ def main():
         line["cpu_baseline"] = cpu_baseline(log_n, threads, j_sample=512, gpu_out=gpu_out)
         line["bit_exact_vs_cpu"] = line["cpu_baseline"].get("bit_exact_vs_gpu")
         # the same harness path run in full at 2^16, beside its sampled estimate
-        _, _, out16 = time_plan(field, 16, 3, 1, want_output=True)
+        _, _, out16 = time_plan(field, 16, 3, 1, want_output=True, settle_ms=0.0)
         full16 = cpu_full_run(16, threads, field, out16)
         est16 = cpu_baseline(16, threads, field, serial=False)
         full16["sampled_estimate_seconds"] = est16["seconds_extrapolated"]
