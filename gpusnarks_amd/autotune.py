@@ -48,10 +48,11 @@ def candidates(field, log_n):
             out.append([F32_SINGLE[log_n]])
         for P in (2, 3, 4):
             for comp in compositions(log_n, P, 5, 10):
-                if all(log_n - r >= 4 for r in comp):
-                    out.append([F32_MULTI[r] for r in comp])
-                if all(log_n - r >= 5 and r <= 9 for r in comp):
-                    out.append([F32_WIDE[r] for r in comp])
+                # each pass on its own: 16- or 32-column tiles
+                shapes = [[s for s in (F32_MULTI.get(r), F32_WIDE.get(r)) if s and log_n - r >= s[1]]
+                          for r in comp]
+                for sched in itertools.product(*shapes):
+                    out.append(list(sched))
         return out
     for P in (1, 2, 3, 4):
         for comp in compositions(log_n, P, 3 if P > 1 else 1, 9):
@@ -64,18 +65,35 @@ def candidates(field, log_n):
     return out
 
 
-def time_schedule(field, log_n, sched, x_dev, y_dev, stream, reps):
+# GPU time of untimed transforms before each candidate is timed: the engine
+# clock rises over the first ~25 ms of sustained load and falls again while
+# the host compares the previous candidate's output
+SETTLE_MS = 50.0
+
+
+def time_schedule(field, log_n, sched, x_dev, y_dev, stream, reps, settle_ms=SETTLE_MS):
+    """Mean time of one transform under `sched`: untimed transforms until
+    `settle_ms` of GPU time has passed, then at least `reps` timed back to
+    back (more for short transforms, so the timed region lasts ~10 ms)."""
     g.tuning_set(field, log_n, sched)
     plan = g.Plan(field, log_n)
-    plan.execute(x_dev.ptr, y_dev.ptr, stream.handle)  # warm-up
-    stream.synchronize()
     e0, e1 = g.Event(), g.Event()
-    e0.record(stream.handle)
-    for _ in range(reps):
-        plan.execute(x_dev.ptr, y_dev.ptr, stream.handle)
-    e1.record(stream.handle)
-    e1.synchronize()
-    ms = g.Event.elapsed_ms(e0, e1) / reps
+
+    def timed(k):
+        e0.record(stream.handle)
+        for _ in range(k):
+            plan.execute(x_dev.ptr, y_dev.ptr, stream.handle)
+        e1.record(stream.handle)
+        e1.synchronize()
+        return g.Event.elapsed_ms(e0, e1)
+
+    per = timed(1)
+    spent = per
+    while spent < settle_ms:
+        k = max(1, min(1000, int(5.0 / max(per, 1e-3))))
+        spent += timed(k)
+    k = max(reps, min(2000, int(10.0 / max(per, 1e-3))))
+    ms = timed(k) / k
     plan.close()
     return ms
 
@@ -129,12 +147,30 @@ def batch_lines(path):
         return [l.rstrip("\n") for l in fh if l.split("#", 1)[0].split()[:1] == ["batch"]]
 
 
+def kept_lines(path, results):
+    """The lines of an existing table for sizes this run did not tune, by
+    (field, log2 n)."""
+    out = {}
+    if not os.path.exists(path):
+        return out
+    with open(path) as fh:
+        for l in fh:
+            tok = l.split("#", 1)[0].split()
+            if len(tok) >= 3 and tok[0] != "batch" and (int(tok[0]), int(tok[1])) not in results:
+                out[(int(tok[0]), int(tok[1]))] = l.rstrip("\n")
+    return out
+
+
 def write_table(results, path, keep_from=None):
+    """Tuned sizes from `results`, the other sizes and the batch lines as
+    the existing table (`keep_from`, else `path`) has them."""
     lines = ["# field log_n  log_r,log_w,log_e per pass  (written by gpusnarks_amd.autotune)",
              "# 'batch field log_n ...': the schedule of transforms fused into batches, where it differs"]
-    for (field, log_n), (sched, ms, ms_default) in sorted(results.items()):
-        lines.append("%d %d %s  # %.4f ms (default %.4f ms)"
-                     % (field, log_n, " ".join("%d,%d,%d" % s for s in sched), ms, ms_default))
+    rows = kept_lines(keep_from or path, results)
+    for (field, log_n), (sched, ms, ms_default) in results.items():
+        rows[(field, log_n)] = ("%d %d %s  # %.4f ms (default %.4f ms)"
+                                % (field, log_n, " ".join("%d,%d,%d" % s for s in sched), ms, ms_default))
+    lines += [rows[k] for k in sorted(rows)]
     lines += batch_lines(keep_from or path)
     with open(path, "w") as fh:
         fh.write("\n".join(lines) + "\n")

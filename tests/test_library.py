@@ -166,3 +166,33 @@ def test_tuning_table_alone_and_batch_lines(tmp_path):
     finally:
         g.tuning_clear()
         g.tuning_load(shipped)
+
+
+def test_autotuner_candidates_and_table_merge(tmp_path):
+    """The autotuner's 32-bit candidates mix 16- and 32-column passes and
+    all pass the library's schedule check; writing a table for a few sizes
+    keeps every other size and the batch lines of the existing table."""
+    from gpusnarks_amd import autotune as at
+    try:
+        for log_n in (18, 26, 28):
+            cands = at.candidates(g.FIELD_FP32, log_n)
+            widths = {tuple(s[1] for s in c) for c in cands}
+            assert any(len(set(w)) > 1 for w in widths), log_n  # mixed tile widths
+            for c in cands:
+                g.tuning_set(g.FIELD_FP32, log_n, c)  # raises on a shape the library refuses
+        old = tmp_path / "old.txt"
+        old.write_text("# header\n0 20 6,4,4 6,4,4 8,4,4  # kept\n0 26 8,5,4 9,5,4 9,5,4  # replaced\n"
+                       "batch 2 18 9,0,1 9,0,1  # fused\n")
+        new = tmp_path / "new.txt"
+        at.write_table({(g.FIELD_FP32, 26): ([(9, 5, 4), (8, 5, 4), (9, 5, 4)], 0.4, 0.5)}, str(new),
+                       keep_from=str(old))
+        rows = [l for l in new.read_text().splitlines() if not l.startswith("#")]
+        assert rows[0] == "0 20 6,4,4 6,4,4 8,4,4  # kept"
+        assert rows[1].startswith("0 26 9,5,4 8,5,4 9,5,4  #")
+        assert rows[2] == "batch 2 18 9,0,1 9,0,1  # fused"
+        g.tuning_clear()
+        g.tuning_load(str(new))
+        assert g.tuning_get(g.FIELD_FP32, 26) == [(9, 5, 4), (8, 5, 4), (9, 5, 4)]
+    finally:
+        g.tuning_clear()
+        g.tuning_load(os.path.join(os.path.dirname(g.library_path), "tuning.txt"))
