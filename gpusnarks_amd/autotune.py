@@ -28,6 +28,7 @@ DEFAULT_SIZES = {"fp32": range(10, 29), "mnt4": range(10, 16), "mnt6": range(10,
 F32_SINGLE = {r: (r, 0, min(4, r)) for r in range(1, 13)}
 F32_MULTI = {r: (r, 4, 4) for r in range(5, 11)}
 F32_WIDE = {r: (r, 5, 4) for r in range(5, 10)}
+F32_WIDE32 = {r: (r, 5, 5) for r in (9, 10)}  # 32 elements per thread
 
 
 def compositions(total, parts, lo, hi):
@@ -49,8 +50,8 @@ def candidates(field, log_n):
         for P in (2, 3, 4):
             for comp in compositions(log_n, P, 5, 10):
                 # each pass on its own: 16- or 32-column tiles
-                shapes = [[s for s in (F32_MULTI.get(r), F32_WIDE.get(r)) if s and log_n - r >= s[1]]
-                          for r in comp]
+                shapes = [[s for s in (F32_MULTI.get(r), F32_WIDE.get(r), F32_WIDE32.get(r))
+                           if s and log_n - r >= s[1]] for r in comp]
                 for sched in itertools.product(*shapes):
                     out.append(list(sched))
         return out

@@ -4,13 +4,15 @@
 namespace gsn {
 
 // Compiled tile shapes (log_r, log_w, log_e): single-pass transforms up to
-// 2^12 with one column, 16-column passes of 2^5..2^10 rows and 32-column
-// passes (128-byte rows) of 2^5..2^9 rows.
+// 2^12 with one column, 16-column passes of 2^5..2^10 rows, 32-column
+// passes (128-byte rows) of 2^5..2^9 rows with 16 elements per thread, and
+// 32-column passes of 2^9 and 2^10 rows with 32 elements per thread (a
+// 2^10 x 32 tile in 1024 threads, 135 KB of LDS).
 #define GSN_F32_SHAPES(X)                                                                 \
     X(1, 0, 1) X(2, 0, 2) X(3, 0, 3) X(4, 0, 4) X(5, 0, 4) X(6, 0, 4) X(7, 0, 4) X(8, 0, 4) \
     X(9, 0, 4) X(10, 0, 4) X(11, 0, 4) X(12, 0, 4) X(5, 4, 4) X(6, 4, 4) X(7, 4, 4)        \
     X(8, 4, 4) X(9, 4, 4) X(10, 4, 4) X(5, 5, 4) X(6, 5, 4) X(7, 5, 4) X(8, 5, 4)          \
-    X(9, 5, 4)
+    X(9, 5, 4) X(9, 5, 5) X(10, 5, 5)
 
 static inline int shape_key(const PassCfg &c) { return c.log_r * 100 + c.log_w * 10 + c.log_e; }
 

@@ -188,7 +188,8 @@ SCHEDULES = [
                               [(9, 1, 1), (3, 7, 1)], [(10, 0, 1), (2, 8, 1)], [(4, 3, 1)] * 3, [(2, 5, 1)] * 6]),
     (g.FIELD_MNT4753_FR, 13, [[(9, 0, 1), (4, 5, 1)], [(5, 5, 1), (8, 2, 1)]]),
     (g.FIELD_FP32, 16, [[(8, 4, 4), (8, 4, 4)], [(8, 5, 4), (8, 5, 4)], [(6, 5, 4), (5, 5, 4), (5, 5, 4)],
-                        [(9, 4, 4), (7, 4, 4)], [(10, 4, 4), (6, 4, 4)]]),
+                        [(9, 4, 4), (7, 4, 4)], [(10, 4, 4), (6, 4, 4)], [(10, 5, 5), (6, 5, 4)],
+                        [(6, 5, 4), (10, 5, 5)], [(9, 5, 5), (7, 5, 4)], [(7, 5, 4), (9, 5, 5)]]),
 ]
 
 
@@ -196,7 +197,8 @@ SCHEDULES = [
 def test_every_tile_shape_matches_oracle(orc, field, log_n, schedules):
     """Explicit pass schedules covering every 753-bit workgroup size (64..512
     threads), wide and one-column tiles, and the 16- and 32-column 32-bit
-    shapes: each bit-exact against the oracle, forward and inverse."""
+    shapes (16 or 32 elements per thread, first or twiddled pass): each
+    bit-exact against the oracle, forward and inverse."""
     x = inputs(field, log_n, 0x5C4ED + log_n)
     om = g.root_of_unity(field, log_n)
     want = orc.serial_fft(field, x, om)
