@@ -117,7 +117,8 @@ def time_plan(field, log_n, steps, warmup, inverse=False, want_output=False, set
 
 def step_times(field, log_n, runs=20, warmup=3):
     """Per-transform durations (HIP events around each execute, operands in
-    HBM): median and min over `runs` transforms."""
+    HBM): median and min over `runs` transforms, after `warmup` transforms
+    and SETTLE_MS of untimed GPU time."""
     import gpusnarks_amd as g
     x = inputs(field, log_n, 0x5EED0000 + log_n)
     plan = g.Plan(field, log_n)
@@ -126,6 +127,15 @@ def step_times(field, log_n, runs=20, warmup=3):
     s = g.Stream()
     for _ in range(warmup):
         plan.execute(d_in.ptr, d_out.ptr, s.handle)
+    e0, e1 = g.Event(), g.Event()
+    spent = 0.0
+    while spent < SETTLE_MS:
+        e0.record(s.handle)
+        for _ in range(runs):
+            plan.execute(d_in.ptr, d_out.ptr, s.handle)
+        e1.record(s.handle)
+        e1.synchronize()
+        spent += g.Event.elapsed_ms(e0, e1)
     ev = [g.Event() for _ in range(runs + 1)]
     ev[0].record(s.handle)
     for i in range(runs):
