@@ -263,9 +263,11 @@ int gsn_msm(gsn_msm_bases *bases, void *d_out, const void *d_scalars, void *stre
 /* --- autotuning: per-size pass schedules -----------------------------------
  * A schedule is num_passes triples (log_r, log_w, log_e): pass radix 2^log_r,
  * 2^log_w columns per workgroup, register radix 2^log_e; the log_r sum to
- * log_n.  Setting one drops cached plans.  At start-up the library reads
- * tuning.txt beside itself (or $GSN_TUNING_FILE): lines
- * "field log_n r,w,e r,w,e ...". */
+ * log_n.  Setting one drops cached plans from the cache (plans already in
+ * use, by batches or by calls running on other threads, stay valid).  At
+ * start-up the library reads tuning.txt beside itself (or $GSN_TUNING_FILE):
+ * lines "field log_n r,w,e r,w,e ...", and "batch field log_n ..." for the
+ * schedule of transforms fused into batches. */
 int gsn_tuning_set(int field, unsigned log_n, const int *shapes, int num_passes);
 int gsn_tuning_get(int field, unsigned log_n, int *shapes, int max_passes, int *num_passes);
 int gsn_tuning_clear(void);
