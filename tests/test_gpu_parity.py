@@ -232,3 +232,38 @@ def test_fp32_direct_pass_table_matches_oracle(orc, log_n, schedule):
         assert np.array_equal(g.best_ifft(got, om, field), x)
     finally:
         g.tuning_set(field, log_n, before)
+
+
+def test_host_entry_from_several_threads(orc):
+    """The best_fft entry point called from 6 host threads at once (ctypes
+    releases the GIL): two threads share each (field, size) plan, and one
+    of them retunes that size halfway, which drops the cached plan while the
+    other may still be using it.  Every output equals the oracle's."""
+    from concurrent.futures import ThreadPoolExecutor
+    cases = [(g.FIELD_MNT6753_FR, 12), (g.FIELD_FP32, 16), (g.FIELD_MNT4753_FR, 11)]
+    jobs = []
+    for c, (field, log_n) in enumerate(cases):
+        for t in range(2):
+            x = inputs(field, log_n, 0x7A4E + 16 * c + t)
+            om = g.root_of_unity(field, log_n)
+            jobs.append((field, log_n, x, om, orc.serial_fft(field, x, om)))
+    befores = {(f, l): g.tuning_get(f, l) for f, l in cases}
+
+    def work(k):
+        field, log_n, x, om, want = jobs[k]
+        for rep in range(4):
+            if k % 2 == 1 and rep == 2:
+                g.tuning_set(field, log_n, befores[(field, log_n)])  # same schedule, fresh plans
+            got = g.best_fft(x, om, field)
+            if not np.array_equal(got, want):
+                return (k, rep)
+        return None
+
+    say("host entry: %d threads" % len(jobs))
+    try:
+        with ThreadPoolExecutor(len(jobs)) as ex:
+            bad = [r for r in ex.map(work, range(len(jobs))) if r is not None]
+    finally:
+        for (f, l), sched in befores.items():
+            g.tuning_set(f, l, sched)
+    assert not bad, bad
